@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark: SPH-MHD particle-steps/s at 2M particles on one MI355X, plus ensemble step latency.
+
+Workload (the metric's configuration): mixed-precision 3D MHD turbulence in a periodic unit cube,
+128^3 = 2,097,152 particles (jittered lattice, rho = P = 1, gamma = 5/3, Gaussian random velocity
+with E(k) ~ k^-2 for k in [1, 8] at rms Mach 1, B = 0.1 z + 10% divergence-free perturbation),
+Wendland C2 with ~57 neighbours, Monaghan viscosity, SPMHD with Borve correction and hyperbolic
+cleaning, CFL dt. A step = drift + resort + density + forces + kick with FP64 CFL/conservation
+reductions, all on the GPU with inputs resident in HBM.
+
+Prints one JSON line. value = particles * K / wall time of K steps (bracketed by device syncs).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from spearhed_amd import ic  # noqa: E402
+from spearhed_amd.stepper import Context  # noqa: E402
+
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (MFMA unused: pair physics is not a dense contraction)
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic FP32 work per accepted pair (+,-,*,/,sqrt,rsqrt,min/max/compare = 1; FMA = 2), counted
+# from the unconditional arithmetic of the force and density pair bodies (the viscosity branch for
+# approaching pairs is not counted, so this is a lower bound); per-target epilogue work per particle.
+FORCE_FLOP_PER_PAIR = 96
+FORCE_FLOP_PER_PARTICLE = 40
+DENSITY_FLOP_PER_PAIR = 23
+# Algorithmic HBM bytes of the force pass per particle: read the 5 float4 interaction records once,
+# write dv/dt (12), du/dt, dB/dt (12), dpsi/dt, div B, v_sig (16).
+FORCE_BYTES_PER_PARTICLE = 80 + 44
+
+
+def run_main(args):
+    t0 = time.perf_counter()
+    cfg, ps = ic.mhd_turbulence(n=args.n, seed=3)
+    n = ps.n
+    t_ic = time.perf_counter() - t0
+    ctx = Context([cfg], [n], precision=args.precision)
+    ctx.upload(0, ps)
+    if args.tune:
+        ctx.prime()
+        ctx.autotune(2)
+    ctx.prime()
+    for _ in range(args.warmup):
+        ctx.step(1)
+    ctx.sync()
+    ctx.profile_begin(args.steps)
+    t1 = time.perf_counter()
+    ctx.step(args.steps)
+    ctx.sync()
+    t2 = time.perf_counter()
+    prof = ctx.profile_end()
+    wall = t2 - t1
+    pairs = ctx.neighbour_total(0)
+    cons = ctx.conserved()[0]
+    tuning = ctx.tuning
+    ctx.close()
+    return dict(n=n, wall=wall, prof=prof, pairs=pairs, cons=cons, t_ic=t_ic, cfg=cfg, ps=ps, tuning=tuning)
+
+
+def run_ensemble(args):
+    runs = ic.kh_ensemble(args.ensemble_runs, seed=4)
+    sizes = [p.n for _, p in runs]
+    ctx = Context([c for c, _ in runs], sizes, precision="mixed")
+    for r, (_, p) in enumerate(runs):
+        ctx.upload(r, p)
+    ctx.prime()
+    ctx.step(2)
+    ctx.sync()
+    k = args.ensemble_steps
+    ctx.profile_begin(k)
+    t1 = time.perf_counter()
+    ctx.step(k)
+    ctx.sync()
+    wall = time.perf_counter() - t1
+    prof = ctx.profile_end()
+    ctx.close()
+    total = sum(sizes)
+    return {"runs": len(runs), "particles": total, "steps": k, "p50_ms": prof["step_ms_p50"],
+            "p99_ms": prof["step_ms_p99"], "particle_steps_per_s": total * k / wall}
+
+
+def cpu_baseline(cfg, ps, budget_s: float):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ref
+
+    o = oracle_ref.OracleRun(cfg, ps, "mixed")
+    o.prime()
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 20:
+            break
+    return {"value": ps.n * steps / el, "unit": "particle-steps/s", "cores": int(oracle_ref.lib().orc_num_threads()),
+            "kind": "port",
+            "sample": f"{steps} full steps of the same 2,097,152-particle workload (C restatement of the "
+                      f"reference CPU arithmetic, OpenMP over supercells), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=128, help="lattice points per axis (128 -> 2,097,152)")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32"])
+    ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-runs", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (the step would shard by "
+              "supercell bricks with halo exchange; see the design notes). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    m = run_main(args)
+    n, K = m["n"], args.steps
+    value = n * K / m["wall"]
+    f_ms = m["prof"]["stage_ms"]["forces"]
+    pairs = m["pairs"]
+    force_flop = FORCE_FLOP_PER_PAIR * pairs + FORCE_FLOP_PER_PARTICLE * n
+    achieved = force_flop / (f_ms * 1e-3) / 1e12
+    out = {
+        "metric": "SPH-MHD particle-steps/s at 2M particles; ensemble p50/p99 step latency",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": m["wall"] * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair arithmetic, f64 positions/reductions" if args.precision == "mixed" else "f32",
+        "data": "synthetic (seeded jittered lattice, Gaussian random velocity and B fields)",
+        "config": {"workload": f"MHD turbulence {args.n}^3 periodic cube, {args.precision} precision, "
+                               "Wendland C2 ~57 neighbours, viscosity + SPMHD + cleaning, CFL dt",
+                   "particles": n, "precision": args.precision, "tuning": m["tuning"]},
+        "stage_ms": m["prof"]["stage_ms"],
+        "step_ms_p50": m["prof"]["step_ms_p50"],
+        "step_ms_p99": m["prof"]["step_ms_p99"],
+        "neighbours_mean": pairs / n,
+        "roofline": {
+            "kernel": "k_forces (force pass)",
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "flop_per_launch": force_flop,
+            "launch_ms": f_ms,
+            "algorithmic_bytes_per_launch": FORCE_BYTES_PER_PARTICLE * n,
+            "hbm_frac": FORCE_BYTES_PER_PARTICLE * n / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        },
+        "conserved": m["cons"],
+    }
+    if not args.no_ensemble:
+        out["ensemble"] = run_ensemble(args)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(m["cfg"], m["ps"], args.cpu_seconds)
+        out["vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

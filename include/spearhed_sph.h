@@ -1,0 +1,201 @@
+/*
+ * spearhed_sph.h -- C ABI of the MI355X (gfx950) SPH-MHD particle step.
+ *
+ * This is the drop-in boundary for spearHED's particle step. The reference exposes its stages as
+ * header-only C++ functor -> sender templates; this header exposes the same stages as plain C entry
+ * points over a device-resident context, so that a C++ (or C / ctypes) host can drive them.
+ *
+ * Stage <-> reference mapping (paths relative to the spearHED tree):
+ *   sph_stage_drift            PushVelocity<S> via launchForEach
+ *                              (include/spearhed/particles/pusher/PushVelocity.hpp:35-42,
+ *                               packages/pmacc/include/spmacc/particles/algorithms/LaunchForEach.hpp:154-159)
+ *   sph_stage_resort           UpdateVolumes<PR> + FrameIndexBuffer::refreshIfStaleSender
+ *                              (packages/pmacc/include/spmacc/particles/regions/RegionBoundsUpdate.hpp:175-214,
+ *                               packages/pmacc/include/spmacc/particles/algorithms/FrameIndex.hpp:271-274)
+ *   sph_stage_neighbour_counts calculateNeighboursSender (per-particle form of the region CSR)
+ *                              (packages/pmacc/include/spmacc/particles/regions/NeighbourRegions.hpp:233-255)
+ *   sph_stage_density          UpdateDensity<K>  (include/spearhed/particles/density/DensitySummation.hpp:104-138)
+ *   sph_stage_forces           UpdateHydroForces<K> (include/spearhed/sph/HydroForces.hpp:183-214), extended
+ *                              with artificial viscosity, SPMHD stress/induction and div-B cleaning
+ *   sph_stage_integrate        EulerIntegrate (include/spearhed/particles/pusher/EulerIntegrate.hpp:43-52),
+ *                              extended to B and psi, fused with the CFL and conservation reductions
+ *   sph_step                   Simulation::runOneStep (include/spearhed/control/Simulation.x.cpp:196-310)
+ *   sph_sync                   caravan::syncWait at the end of runOneStep (Simulation.x.cpp:303-307)
+ *
+ * Conventions: every call returns 0 on success or a negative SPH_E_* code; sph_last_error() gives
+ * the message of the last failure on a context. All stage calls are asynchronous on the context's
+ * stream until sph_sync(). A context is not thread-safe; distinct contexts may be driven from
+ * different host threads. The context owns its device memory; the caller owns host arrays.
+ */
+#ifndef SPEARHED_SPH_H
+#define SPEARHED_SPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------------------------------- */
+#define SPH_OK 0
+#define SPH_E_INVALID (-1)   /* bad argument / configuration */
+#define SPH_E_NOMEM (-2)     /* device allocation failed or HBM budget exceeded */
+#define SPH_E_DEVICE (-3)    /* HIP runtime error */
+#define SPH_E_STATE (-4)     /* call not valid in the context's current state */
+#define SPH_E_CAPACITY (-5)  /* a fixed capacity (LDS tile, run table) was exceeded */
+
+/* ---- smoothing kernels (include/spearhed/sph/SphKernel.hpp:45-51 concept) --------------------- */
+#define SPH_KERNEL_CUBIC 0     /* M4, support 2h   (include/spearhed/sph/CubicSplineKernel.hpp:40-162) */
+#define SPH_KERNEL_QUINTIC 1   /* M5, support 3h   (include/spearhed/sph/QuinticSplineKernel.hpp:39-186) */
+#define SPH_KERNEL_WENDLAND 2  /* Wendland C2, support 2h (Wendland 1995; Dehnen & Aly 2012) */
+
+/* ---- precision modes ------------------------------------------------------------------------ */
+#define SPH_PREC_FP32 0   /* plain: FP32 absolute positions, FP32 drift, FP32 pair arithmetic */
+#define SPH_PREC_MIXED 1  /* FP64 positions/drift/rebase, FP32 pair arithmetic on supercell-relative
+                             coordinates, FP64 min-dt and conservation sums */
+
+/* ---- species (include/spearhed/param/speciesDefinition.param:56-88) ------------------------- */
+#define SPH_SPECIES_DEFAULT 0   /* Interior, Movable, Source, Thermodynamic: the target of every pass */
+#define SPH_SPECIES_BOUNDARY 1  /* Frozen, Source, Thermodynamic: walls, a source only */
+#define SPH_SPECIES_TRACER 2    /* Movable only: drifted with its own velocity, never a source */
+
+/* Per-run physics and geometry. One run = one independent simulation box. */
+typedef struct sph_run_config {
+    double box_lo[3];     /* lower corner of the box */
+    double box_len[3];    /* edge lengths; periodic axes wrap with this period */
+    int32_t periodic[3];  /* 1 = periodic axis (minimum image through the supercell stencil) */
+    int32_t kernel;       /* SPH_KERNEL_* */
+    float h0;             /* smoothing length that sets the interaction radius H = support * h0
+                             (reference: sph.param h0, Simulation.x.cpp:263-264) */
+    float gamma;          /* adiabatic index (EquationOfState.hpp:32-47) */
+    float alpha_visc;     /* Monaghan artificial viscosity alpha (0 = inviscid, as the reference) */
+    float beta_visc;      /* Monaghan artificial viscosity beta; also the signal-velocity coefficient */
+    int32_t mhd;          /* 1 = SPMHD (Lorentz force, induction); 0 = hydro only */
+    int32_t cleaning;     /* 1 = Tricco & Price (2012) constrained hyperbolic div-B cleaning */
+    int32_t borve;        /* 1 = Borve et al. (2001) tensile-instability correction */
+    float clean_sigma;    /* cleaning damping: tau = h / (clean_sigma * c_h) */
+    float cfl;            /* Courant factor on h / v_sig (reference sph.param cflFactor) */
+    float cfl_force;      /* factor on sqrt(h / |a|) */
+    double dt_fixed;      /* > 0: constant dt (reference dimension.param dt); <= 0: CFL dt */
+    float cell_factor;    /* supercell edge >= cell_factor * H (>= 1) */
+    int32_t reserved;
+} sph_run_config;
+
+/* Host-side SoA view of one run, named after the reference attributes
+ * (include/spearhed/param/speciesDefinition.param:100-110). Unused pointers may be NULL where
+ * noted. Positions are absolute coordinates in FP64 on the host side. */
+typedef struct sph_host_soa {
+    int64_t n;
+    uint64_t* id;          /* particleId */
+    uint8_t* mask;         /* multiMask: 1 = live */
+    uint8_t* species;      /* SPH_SPECIES_* */
+    double* pos[3];        /* position (absolute) */
+    float* mass;
+    float* vel[3];
+    float* density;
+    float* smoothing_length;
+    float* internal_energy;
+    float* bfield[3];      /* magnetic field B (may be NULL for hydro runs: zero) */
+    float* psi;            /* cleaning scalar (may be NULL: zero) */
+    /* derivative outputs (download only; may be NULL) */
+    float* dvdt[3];
+    float* dudt;
+    float* dbdt[3];
+    float* dpsidt;
+    float* divb;
+    float* vsig;
+    uint32_t* n_neighbours;
+} sph_host_soa;
+
+/* FP64 global reductions of one run (new in this path: the reference has none on its step). */
+typedef struct sph_conserved {
+    double mass;
+    double momentum[3];
+    double energy;   /* sum m (v^2/2 + u + B^2/(2 rho)) */
+    double dt_next;  /* CFL dt for the next step */
+    double time;     /* simulated time of the run */
+    double c_h;      /* cleaning speed used in the last force pass */
+} sph_conserved;
+
+/* Grid description of one run (queryable for parity checks). */
+typedef struct sph_grid_info {
+    int32_t n[3];          /* supercells per axis */
+    int32_t n_cells;       /* n[0]*n[1]*n[2] */
+    double cell[3];        /* supercell edge per axis */
+    float radius;          /* interaction radius H */
+    int64_t particle_offset; /* first particle of the run in the context's arrays */
+    int64_t cell_offset;     /* first supercell of the run in the context's cell table */
+} sph_grid_info;
+
+/* Launch tunables chosen by the autotuner (sph_autotune). */
+typedef struct sph_tuning {
+    int32_t density_chunk;  /* neighbours staged in LDS per chunk by the density pass: 128|256|512 */
+    int32_t force_chunk;    /* neighbours staged in LDS per chunk by the force pass: 64|128|256 */
+    int32_t waves_per_cu;   /* persistent-grid waves per CU: 4..32 */
+    int32_t max_lanes_per_target; /* cap on K, the lanes that split one target's candidates: 1..64 */
+} sph_tuning;
+
+typedef struct sph_ctx sph_ctx;
+
+/* ---- context -------------------------------------------------------------------------------- */
+/* Create a context holding n_runs independent runs (1 = single run; >1 = ragged batch).
+ * capacity[r] = particles of run r. stream: a hipStream_t (NULL = a new stream owned by the ctx).
+ * Replaces Simulation::init's buffer set-up (include/spearhed/control/Simulation.x.cpp:312-352). */
+int sph_ctx_create(const sph_run_config* runs, const int64_t* capacity, int32_t n_runs,
+                   int32_t precision, void* stream, sph_ctx** out);
+int sph_ctx_destroy(sph_ctx* ctx);
+const char* sph_last_error(const sph_ctx* ctx);
+int sph_sync(sph_ctx* ctx);
+int sph_get_grid(sph_ctx* ctx, int32_t run, sph_grid_info* out);
+int sph_set_tuning(sph_ctx* ctx, const sph_tuning* t);
+int sph_get_tuning(sph_ctx* ctx, sph_tuning* out);
+/* Bytes of HBM a run of n particles needs at a precision (admission by HBM budget). */
+int64_t sph_bytes_per_run(const sph_run_config* cfg, int64_t n, int32_t precision);
+
+/* ---- data movement (the caller owns the host arrays) ---------------------------------------- */
+int sph_upload(sph_ctx* ctx, int32_t run, const sph_host_soa* src);
+/* Downloads in the context's current (supercell-sorted) order; compare by id, never by position. */
+int sph_download(sph_ctx* ctx, int32_t run, sph_host_soa* dst);
+
+/* ---- stages (Simulation::runOneStep order: drift, resort, density, forces, integrate) -------- */
+int sph_stage_drift(sph_ctx* ctx, double dt);      /* dt <= 0: use each run's device-resident CFL dt */
+int sph_stage_resort(sph_ctx* ctx);
+int sph_stage_neighbour_counts(sph_ctx* ctx);      /* fills n_neighbours (download to read) */
+int sph_stage_density(sph_ctx* ctx);
+int sph_stage_forces(sph_ctx* ctx);
+int sph_stage_integrate(sph_ctx* ctx, double dt);  /* dt <= 0: device-resident CFL dt */
+/* Reductions over the current state: per-run CFL dt from the last force pass, and FP64 sums. */
+int sph_reduce_min_dt(sph_ctx* ctx, double* dt_per_run);
+int sph_reduce_conserved(sph_ctx* ctx, sph_conserved* per_run);
+/* Resort + density + forces + CFL reduction on the initial state: sets each run's first dt. */
+int sph_prime(sph_ctx* ctx);
+/* Whole steps: drift, resort, density, forces, integrate (+ reductions), n_steps times. */
+int sph_step(sph_ctx* ctx, int32_t n_steps);
+/* Capture one whole step of every run into a hipGraph and replay it. */
+int sph_graph_capture(sph_ctx* ctx);
+int sph_graph_launch(sph_ctx* ctx, int32_t n_steps);
+/* Per-particle sort results of the last resort (run-local, current order). */
+int sph_get_sort(sph_ctx* ctx, int32_t run, uint32_t* cell_key, uint32_t* cell_start);
+
+/* ---- measurement ---------------------------------------------------------------------------- */
+/* Per-stage HIP-event timing of the next steps (sph_step), recorded on the context's stream; the
+ * analogue of the reference's per-step TimeInterval report (SimulationHelper.cpp:192-195). */
+#define SPH_NUM_STAGES 5  /* drift+bin, resort, density, forces, integrate */
+typedef struct sph_profile {
+    int32_t steps;                    /* steps recorded */
+    double stage_ms[SPH_NUM_STAGES];  /* mean per step */
+    double step_ms_mean, step_ms_p50, step_ms_p99;
+} sph_profile;
+int sph_profile_begin(sph_ctx* ctx, int32_t max_steps);
+/* Synchronises; step_ms (may be NULL) receives the per-step durations. */
+int sph_profile_end(sph_ctx* ctx, sph_profile* out, double* step_ms);
+
+/* ---- autotuner ------------------------------------------------------------------------------ */
+/* Time n_steps steps for each candidate tuning on the context's current data (state restored
+ * afterwards) and keep the fastest. Writes the chosen tuning to *best if non-NULL. */
+int sph_autotune(sph_ctx* ctx, int32_t n_steps, sph_tuning* best);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPEARHED_SPH_H */

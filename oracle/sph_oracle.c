@@ -1,0 +1,149 @@
+/*
+ * sph_oracle.c -- CPU oracle for the SPH(-MHD) particle step. TEST INFRASTRUCTURE ONLY: loaded by
+ * tests, the smoke check and the benchmark's CPU-baseline leg; never by the product library.
+ *
+ * Compile with -ffp-contract=off so that every a*b+c rounds twice, as the restated reference code
+ * does on its CPU backends; the binning and neighbour-count contract with the HIP path depends on it.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "sph_oracle.h"
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* Uniform supercell grid: per axis the largest cell count whose edge is >= cell_factor * H with a
+ * 1e-4 relative margin (so FP32 rounding of the cell index can never drop a pair within H).
+ * This replaces the reference's irregular region AABBs and their brute-force overlap search
+ * (packages/pmacc/include/spmacc/particles/regions/NeighbourRegions.hpp:55-135). */
+int orc_make_grid(const sph_run_config* cfg, orc_grid* g)
+{
+    memset(g, 0, sizeof(*g));
+    if (!(cfg->h0 > 0.0f)) return -1;
+    g->support = (cfg->kernel == SPH_KERNEL_QUINTIC) ? 3 : 2;
+    g->H = (float)g->support * cfg->h0;
+    g->H2 = g->H * g->H;
+    g->Hd = (double)g->support * (double)cfg->h0;
+    g->H2d = g->Hd * g->Hd;
+    double f = cfg->cell_factor > 1.0f ? (double)cfg->cell_factor : 1.0;
+    int64_t ncell = 1;
+    for (int a = 0; a < 3; ++a) {
+        double L = cfg->box_len[a];
+        if (!(L > 0)) return -1;
+        double target = (double)g->H * f * (1.0 + 1e-4);
+        if (cfg->periodic[a] && L < (double)g->H * (1.0 + 1e-4)) return -1;
+        int n = (int)floor(L / target);
+        if (n < 1) n = 1;
+        g->n[a] = n;
+        g->lo[a] = cfg->box_lo[a];
+        g->L[a] = L;
+        g->s[a] = L / (double)n;
+        g->inv_s[a] = (double)n / L;
+        g->lo32[a] = (float)g->lo[a];
+        g->L32[a] = (float)L;
+        g->s32[a] = (float)g->s[a];
+        g->inv_s32[a] = (float)g->inv_s[a];
+        ncell *= n;
+    }
+    if (ncell > 0x7fffffffLL) return -1;
+    g->ncell = (int32_t)ncell;
+    return 0;
+}
+
+int orc_bin(const sph_run_config* cfg, int32_t prec, int64_t n, double* const pos[3], uint32_t* key)
+{
+    orc_grid g;
+    if (orc_make_grid(cfg, &g)) return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        int c[3];
+        for (int a = 0; a < 3; ++a) {
+            int ci;
+            if (prec == ORC_PREC_PLAIN) {
+                float x = (float)pos[a][i];
+                float t = (x - g.lo32[a]) * g.inv_s32[a];
+                ci = (int)floorf(t);
+            } else {
+                double t = (pos[a][i] - g.lo[a]) * g.inv_s[a];
+                ci = (int)floor(t);
+            }
+            if (ci < 0) ci = 0;
+            if (ci > g.n[a] - 1) ci = g.n[a] - 1;
+            c[a] = ci;
+        }
+        key[i] = (uint32_t)(c[0] + g.n[0] * (c[1] + g.n[1] * c[2]));
+    }
+    return 0;
+}
+
+static const uint64_t* g_sort_ids;
+static int cmp_by_id(const void* a, const void* b)
+{
+    uint64_t x = g_sort_ids[*(const int64_t*)a], y = g_sort_ids[*(const int64_t*)b];
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* Stable order by (supercell key, particle id): a counting sort over keys, then each cell by id. */
+int orc_sort(int64_t n, const uint32_t* key, const uint64_t* id, int32_t ncell, int64_t* perm,
+             uint32_t* cell_start)
+{
+    memset(cell_start, 0, ((size_t)ncell + 1) * sizeof(uint32_t));
+    for (int64_t i = 0; i < n; ++i) {
+        if (key[i] >= (uint32_t)ncell) return -1;
+        cell_start[key[i] + 1]++;
+    }
+    for (int32_t c = 0; c < ncell; ++c) cell_start[c + 1] += cell_start[c];
+    uint32_t* cur = (uint32_t*)malloc(((size_t)ncell + 1) * sizeof(uint32_t));
+    memcpy(cur, cell_start, ((size_t)ncell + 1) * sizeof(uint32_t));
+    for (int64_t i = 0; i < n; ++i) perm[cur[key[i]]++] = i;
+    free(cur);
+    g_sort_ids = id;
+    for (int32_t c = 0; c < ncell; ++c) {
+        int64_t b = cell_start[c], e = cell_start[c + 1];
+        if (e - b > 1) qsort(perm + b, (size_t)(e - b), sizeof(int64_t), cmp_by_id);
+    }
+    return 0;
+}
+
+/* Role predicates (include/spearhed/param/speciesDefinition.param:56-88): targets are Default,
+ * sources are Default and Boundary, movable are Default and Tracer. */
+static inline int ksupport(int kernel) { return kernel == SPH_KERNEL_QUINTIC ? 3 : 2; }
+
+static inline int is_target(uint8_t sp, uint8_t live) { return live && sp == SPH_SPECIES_DEFAULT; }
+static inline int is_source(uint8_t sp, uint8_t live)
+{
+    return live && (sp == SPH_SPECIES_DEFAULT || sp == SPH_SPECIES_BOUNDARY);
+}
+static inline int is_movable(uint8_t sp, uint8_t live)
+{
+    return live && (sp == SPH_SPECIES_DEFAULT || sp == SPH_SPECIES_TRACER);
+}
+
+static inline void cell_coords(const orc_grid* g, int cell, int c[3])
+{
+    c[0] = cell % g->n[0];
+    c[1] = (cell / g->n[0]) % g->n[1];
+    c[2] = cell / (g->n[0] * g->n[1]);
+}
+
+#define REAL float
+#define S f32
+#include "sph_oracle_impl.h"
+#undef REAL
+#undef S
+
+#define REAL double
+#define S f64
+#include "sph_oracle_impl.h"
+#undef REAL
+#undef S

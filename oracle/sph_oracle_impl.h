@@ -1,0 +1,762 @@
+/*
+ * sph_oracle_impl.h -- body of the CPU oracle (TEST INFRASTRUCTURE ONLY), compiled twice by
+ * sph_oracle.c: REAL=float with S=f32, REAL=double with S=f64. Not a public header.
+ *
+ * The physics follows, per function:
+ *   kernels          include/spearhed/sph/CubicSplineKernel.hpp:47-132, QuinticSplineKernel.hpp:46-160
+ *                    (W, dWdr, gradWScalar; the guard !(r > 0) || r >= support*h -> 0), plus
+ *                    Wendland C2 (Wendland 1995; Dehnen & Aly 2012 MNRAS 425:1068 eq. 5 with H = 2h).
+ *   pair geometry    packages/pmacc/include/spmacc/particles/algorithms/ParticleParticleInteraction.hpp:236-311
+ *                    (pre-shifted staged position, strict r2 < R^2 cull, invR = rsqrt(r2) only
+ *                    when r2 > 0, self pair excluded by identity).
+ *   density          include/spearhed/particles/density/DensitySummation.hpp:50-94 (gather with own h,
+ *                    self term m_i W(0, h_i), coincident non-self pairs contribute m_j W(0, h_i)).
+ *   hydro force      include/spearhed/sph/HydroForces.hpp:112-175 (symmetric pressure gradient,
+ *                    P dV energy with the own-side P_i/rho_i^2), EquationOfState.hpp:32-47.
+ *   viscosity        Monaghan 1997 JCP 136:298 / Monaghan & Gingold 1983 (alpha, beta; mu with
+ *                    eta^2 = 0.01 h^2; arithmetic means of h, c, rho).
+ *   SPMHD            Price 2012 JCP 231:759 sec. 4 (Maxwell-stress momentum equation, B induction
+ *                    with the own-side kernel gradient, difference divergence of B), Borve et al.
+ *                    2001 tensile correction, Tricco & Price 2012 JCP 231:7214 (constrained
+ *                    hyperbolic cleaning: symmetric grad psi, psi' = -c_h^2 div B - psi/tau
+ *                    - psi div v / 2).
+ *   drift / kick     include/spearhed/particles/pusher/PushVelocity.hpp:35-42,
+ *                    EulerIntegrate.hpp:43-52 (symplectic Euler: drift with the old v, kick after
+ *                    the force pass), extended to B and psi.
+ *   step order       include/spearhed/control/Simulation.x.cpp:196-310.
+ */
+
+#define ORC_CAT2(a, b) a##_##b
+#define ORC_CAT(a, b) ORC_CAT2(a, b)
+#define FN(name) ORC_CAT(name, S)
+
+#define RA(p) ((REAL*)(p))
+
+static const REAL FN(kPi) = (REAL)3.14159265358979323846;
+
+/* Kernel normalisation sigma in 3D (CubicSplineKernel.hpp:47-58, QuinticSplineKernel.hpp:46-57). */
+static inline REAL FN(knorm)(int kernel)
+{
+    if (kernel == SPH_KERNEL_CUBIC) return (REAL)1 / FN(kPi);
+    if (kernel == SPH_KERNEL_QUINTIC) return (REAL)1 / ((REAL)120 * FN(kPi));
+    return (REAL)21 / ((REAL)16 * FN(kPi));
+}
+
+/* Shape function f(q), W = sigma/h^3 f(q) (CubicSplineKernel.hpp:67-86,
+ * QuinticSplineKernel.hpp:66-101). */
+static inline REAL FN(kshape)(int kernel, REAL q)
+{
+    if (kernel == SPH_KERNEL_CUBIC) {
+        if (q < (REAL)1) return (REAL)1 - (REAL)1.5 * q * q + (REAL)0.75 * q * q * q;
+        if (q < (REAL)2) {
+            REAL t = (REAL)2 - q;
+            return (REAL)0.25 * t * t * t;
+        }
+        return (REAL)0;
+    }
+    if (kernel == SPH_KERNEL_QUINTIC) {
+        REAL a = (REAL)3 - q, b = (REAL)2 - q, c = (REAL)1 - q;
+        if (q < (REAL)1) return a * a * a * a * a - (REAL)6 * b * b * b * b * b + (REAL)15 * c * c * c * c * c;
+        if (q < (REAL)2) return a * a * a * a * a - (REAL)6 * b * b * b * b * b;
+        if (q < (REAL)3) return a * a * a * a * a;
+        return (REAL)0;
+    }
+    /* Wendland C2: (1 - q/2)^4 (1 + 2q) for q < 2 */
+    if (q < (REAL)2) {
+        REAL t = (REAL)1 - (REAL)0.5 * q;
+        REAL t2 = t * t;
+        return t2 * t2 * ((REAL)1 + (REAL)2 * q);
+    }
+    return (REAL)0;
+}
+
+/* Radial-derivative shape g(q), dW/dr = sigma/h^4 g(q) (CubicSplineKernel.hpp:94-114,
+ * QuinticSplineKernel.hpp:109-143). Wendland: g = -5 q (1 - q/2)^3. */
+static inline REAL FN(kdshape)(int kernel, REAL q)
+{
+    if (kernel == SPH_KERNEL_CUBIC) {
+        if (q >= (REAL)0 && q < (REAL)1) return (REAL)2.25 * q * q - (REAL)3 * q;
+        if (q < (REAL)2) {
+            REAL t = (REAL)2 - q;
+            return -(REAL)0.75 * t * t;
+        }
+        return (REAL)0;
+    }
+    if (kernel == SPH_KERNEL_QUINTIC) {
+        REAL a = (REAL)3 - q, b = (REAL)2 - q, c = (REAL)1 - q;
+        if (q < (REAL)1) return -(REAL)5 * a * a * a * a + (REAL)30 * b * b * b * b - (REAL)75 * c * c * c * c;
+        if (q < (REAL)2) return -(REAL)5 * a * a * a * a + (REAL)30 * b * b * b * b;
+        if (q < (REAL)3) return -(REAL)5 * a * a * a * a;
+        return (REAL)0;
+    }
+    if (q < (REAL)2) {
+        REAL t = (REAL)1 - (REAL)0.5 * q;
+        return -(REAL)5 * q * t * t * t;
+    }
+    return (REAL)0;
+}
+
+/* Per-particle kernel factors: W = wfac f(q); gradient scalar s (grad_i W = s r_ij) is
+ * gfac g(q) invR for the splines and gfac (1 - q/2)^3 for Wendland C2 (gfac = -5 sigma/h^5),
+ * which avoids the 1/r of gradWScalar (CubicSplineKernel.hpp:129-137) exactly. */
+static inline void FN(kfactors)(int kernel, REAL h, REAL* invh, REAL* wfac, REAL* gfac)
+{
+    REAL sig = FN(knorm)(kernel);
+    *invh = (REAL)1 / h;
+    *wfac = sig / (h * h * h);
+    if (kernel == SPH_KERNEL_WENDLAND)
+        *gfac = -(REAL)5 * sig / (h * h * h * h * h);
+    else
+        *gfac = sig / (h * h * h * h);
+}
+
+/* gradWScalar with the reference guard (!(r > 0) || r >= support h -> 0), in q = r/h form. */
+static inline REAL FN(kgrad)(int kernel, REAL rr, REAL invR, REAL q, REAL gfac)
+{
+    REAL sup = (REAL)ksupport(kernel);
+    if (!(rr > (REAL)0) || !(q < sup)) return (REAL)0;
+    if (kernel == SPH_KERNEL_WENDLAND) {
+        REAL t = (REAL)1 - (REAL)0.5 * q;
+        return gfac * (t * t * t);
+    }
+    return gfac * FN(kdshape)(kernel, q) * invR;
+}
+
+double FN(orc_kernel_w)(int32_t kernel, double r, double h)
+{
+    REAL invh, wfac, gfac;
+    FN(kfactors)(kernel, (REAL)h, &invh, &wfac, &gfac);
+    REAL q = (REAL)r * invh;
+    return (double)(wfac * FN(kshape)(kernel, q));
+}
+
+double FN(orc_kernel_dwdr)(int32_t kernel, double r, double h)
+{
+    REAL invh, wfac, gfac;
+    FN(kfactors)(kernel, (REAL)h, &invh, &wfac, &gfac);
+    REAL q = (REAL)r * invh;
+    REAL sig = FN(knorm)(kernel);
+    REAL g4 = sig / ((REAL)h * (REAL)h * (REAL)h * (REAL)h);
+    return (double)(g4 * FN(kdshape)(kernel, q));
+}
+
+/* Pair coordinates used by the interaction passes: FP32 absolute (plain), FP32 relative to the
+ * particle's own supercell origin (mixed), or FP64 absolute (f64). Mirrors relativePos + region
+ * origin (packages/pmacc/include/spmacc/particles/attributes/RelativePosition.hpp:36-37). */
+static void FN(pair_coords)(const orc_grid* g, int32_t prec, const orc_state* st, REAL* pc[3])
+{
+    int64_t n = st->n;
+    for (int a = 0; a < 3; ++a) {
+        for (int64_t i = 0; i < n; ++i) {
+            double x = st->pos[a][i];
+            if (prec == ORC_PREC_MIXED) {
+                double t = (x - g->lo[a]) * g->inv_s[a];
+                int c = (int)floor(t);
+                if (c < 0) c = 0;
+                if (c > g->n[a] - 1) c = g->n[a] - 1;
+                pc[a][i] = (REAL)((x - g->lo[a]) - (double)c * g->s[a]);
+            } else {
+                pc[a][i] = (REAL)x;
+            }
+        }
+    }
+}
+
+/* Neighbour supercell (cell + d) with its staged-position shift; returns 0 if outside a
+ * non-periodic axis. */
+static inline int FN(nb_cell)(const orc_grid* g, int32_t prec, const sph_run_config* cfg,
+                              const int c[3], const int d[3], int* cc, REAL shift[3])
+{
+    int idx[3];
+    for (int a = 0; a < 3; ++a) {
+        int x = c[a] + d[a];
+        int wrap = 0;
+        if (cfg->periodic[a]) {
+            if (x < 0) wrap = -1;
+            else if (x >= g->n[a]) wrap = 1;
+            x -= wrap * g->n[a];
+        } else if (x < 0 || x >= g->n[a]) {
+            return 0;
+        }
+        idx[a] = x;
+        if (prec == ORC_PREC_MIXED)
+            shift[a] = (REAL)d[a] * (REAL)g->s32[a];
+        else if (prec == ORC_PREC_PLAIN)
+            shift[a] = (REAL)wrap * (REAL)g->L32[a];
+        else
+            shift[a] = (REAL)((double)wrap * g->L[a]);
+    }
+    *cc = idx[0] + g->n[0] * (idx[1] + g->n[1] * idx[2]);
+    return 1;
+}
+
+/* Squared distance of own pair coordinate to a staged neighbour, evaluated as
+ * ((dx*dx + dy*dy) + dz*dz) without contraction (ParticleParticleInteraction.hpp:281-289). */
+static inline REAL FN(geom)(const REAL pi[3], REAL* const pc[3], int64_t j, const REAL shift[3], REAL d[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        REAL staged = pc[a][j] + shift[a];
+        d[a] = pi[a] - staged;
+    }
+    REAL r2 = d[0] * d[0];
+    r2 = r2 + d[1] * d[1];
+    r2 = r2 + d[2] * d[2];
+    return r2;
+}
+
+int FN(orc_permute)(orc_state* st, const int64_t* perm)
+{
+    int64_t n = st->n;
+    void* tmp = malloc((size_t)n * 8 + 16);
+    if (!tmp) return -2;
+#define PERMUTE(ptr, T)                                                  \
+    if (ptr) {                                                           \
+        T* src = (T*)(ptr);                                              \
+        T* t = (T*)tmp;                                                  \
+        for (int64_t i = 0; i < n; ++i) t[i] = src[perm[i]];             \
+        memcpy(src, t, (size_t)n * sizeof(T));                           \
+    }
+    PERMUTE(st->id, uint64_t)
+    PERMUTE(st->mask, uint8_t)
+    PERMUTE(st->species, uint8_t)
+    for (int a = 0; a < 3; ++a) {
+        PERMUTE(st->pos[a], double)
+        PERMUTE(st->vel[a], REAL)
+        PERMUTE(st->B[a], REAL)
+        PERMUTE(st->dvdt[a], REAL)
+        PERMUTE(st->dbdt[a], REAL)
+    }
+    PERMUTE(st->mass, REAL)
+    PERMUTE(st->rho, REAL)
+    PERMUTE(st->h, REAL)
+    PERMUTE(st->u, REAL)
+    PERMUTE(st->psi, REAL)
+    PERMUTE(st->dudt, REAL)
+    PERMUTE(st->dpsidt, REAL)
+    PERMUTE(st->divb, REAL)
+    PERMUTE(st->vsig, REAL)
+    PERMUTE(st->nnb, uint32_t)
+#undef PERMUTE
+    free(tmp);
+    return 0;
+}
+
+/* Supercell re-sort (replaces UpdateVolumes / the region AABB refit,
+ * packages/pmacc/include/spmacc/particles/regions/RegionBoundsUpdate.hpp:39-214). */
+int FN(orc_resort)(const sph_run_config* cfg, int32_t prec, orc_state* st, uint32_t* cell_start)
+{
+    orc_grid g;
+    if (orc_make_grid(cfg, &g)) return -1;
+    int64_t n = st->n;
+    uint32_t* key = (uint32_t*)malloc((size_t)n * 4 + 4);
+    int64_t* perm = (int64_t*)malloc((size_t)n * 8 + 8);
+    if (!key || !perm) return -2;
+    orc_bin(cfg, prec, n, st->pos, key);
+    orc_sort(n, key, st->id, g.ncell, perm, cell_start);
+    FN(orc_permute)(st, perm);
+    free(key);
+    free(perm);
+    return 0;
+}
+
+typedef struct FN(prim) {
+    REAL invh, wfac, gfac, c, pr, ph, bdr, psr;
+} FN(prim);
+
+/* Per-particle derived quantities after the density pass: EOS (EquationOfState.hpp:32-47),
+ * P/rho^2 (HydroForces.hpp:112-121, 124-137), fast magnetosonic bound c = sqrt((gamma P + B^2)/rho). */
+static void FN(prims)(const sph_run_config* cfg, const orc_state* st, FN(prim)* pr)
+{
+    int64_t n = st->n;
+    REAL gam = (REAL)cfg->gamma;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        FN(prim) p;
+        memset(&p, 0, sizeof(p));
+        if (is_source(st->species[i], st->mask[i])) {
+            REAL h = RA(st->h)[i];
+            FN(kfactors)(cfg->kernel, h, &p.invh, &p.wfac, &p.gfac);
+            REAL rho = RA(st->rho)[i];
+            REAL u = RA(st->u)[i];
+            REAL P = (gam - (REAL)1) * rho * u;
+            REAL invrho = (REAL)1 / rho;
+            REAL B2 = (REAL)0;
+            if (cfg->mhd) {
+                REAL bx = RA(st->B[0])[i], by = RA(st->B[1])[i], bz = RA(st->B[2])[i];
+                B2 = bx * bx + by * by + bz * bz;
+            }
+            p.bdr = invrho * invrho;
+            p.ph = P * p.bdr;
+            p.pr = (P + (REAL)0.5 * B2) * p.bdr;
+            p.psr = cfg->mhd ? RA(st->psi)[i] * p.bdr : (REAL)0;
+            p.c = (REAL)sqrt((gam * P + B2) * invrho);
+        }
+        pr[i] = p;
+    }
+}
+
+/* Density summation over the 27-supercell stencil (DensitySummation.hpp:50-138 semantics). */
+int FN(orc_density)(const sph_run_config* cfg, int32_t prec, orc_state* st, const uint32_t* cell_start)
+{
+    orc_grid g;
+    if (orc_make_grid(cfg, &g)) return -1;
+    int64_t n = st->n;
+    REAL* pc[3];
+    for (int a = 0; a < 3; ++a) pc[a] = (REAL*)malloc((size_t)n * sizeof(REAL) + 8);
+    FN(pair_coords)(&g, prec, st, pc);
+    const REAL H2 = (prec == ORC_PREC_F64) ? (REAL)g.H2d : (REAL)g.H2;
+    const REAL f0 = FN(kshape)(cfg->kernel, (REAL)0);
+    REAL* rho = RA(st->rho);
+    const REAL* m = RA(st->mass);
+    double chmax = 0.0;
+
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int cell = 0; cell < g.ncell; ++cell) {
+        int c[3];
+        cell_coords(&g, cell, c);
+        int ncc = 0, ccs[27];
+        REAL sh[27][3];
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int d[3] = {dx, dy, dz};
+                    if (FN(nb_cell)(&g, prec, cfg, c, d, &ccs[ncc], sh[ncc])) {
+                        if (dx == 0 && dy == 0 && dz == 0) ccs[ncc] |= 0x80000000u; /* own cell */
+                        ++ncc;
+                    }
+                }
+        for (int64_t i = cell_start[cell]; i < (int64_t)cell_start[cell + 1]; ++i) {
+            if (!is_target(st->species[i], st->mask[i])) {
+                if (st->nnb) st->nnb[i] = 0;
+                continue;
+            }
+            REAL invh, wfac, gfac;
+            FN(kfactors)(cfg->kernel, RA(st->h)[i], &invh, &wfac, &gfac);
+            REAL pi[3] = {pc[0][i], pc[1][i], pc[2][i]};
+            REAL acc = (REAL)0;
+            double sc = 0.0;
+            uint32_t cnt = 0;
+            for (int k = 0; k < ncc; ++k) {
+                int own = (ccs[k] & 0x80000000u) != 0;
+                int cc = ccs[k] & 0x7fffffff;
+                for (int64_t j = cell_start[cc]; j < (int64_t)cell_start[cc + 1]; ++j) {
+                    if (!is_source(st->species[j], st->mask[j])) continue;
+                    REAL d[3];
+                    REAL r2 = FN(geom)(pi, pc, j, sh[k], d);
+                    if (!(r2 < H2)) continue;
+                    if (own && j == i) continue;
+                    ++cnt;
+                    REAL invR = (REAL)0, rr = (REAL)0;
+                    if (r2 > (REAL)0) {
+                        invR = (REAL)1 / (REAL)sqrt(r2);
+                        rr = r2 * invR;
+                    }
+                    REAL q = rr * invh;
+                    REAL t = m[j] * FN(kshape)(cfg->kernel, q);
+                    acc += t;
+                    sc += fabs((double)t);
+                }
+            }
+            REAL self = m[i] * f0;
+            rho[i] = wfac * (acc + self);
+            if (st->nnb) st->nnb[i] = cnt;
+            if (st->sc_rho) st->sc_rho[i] = fabs((double)wfac) * (sc + fabs((double)self));
+        }
+    }
+    for (int a = 0; a < 3; ++a) free(pc[a]);
+
+    /* cleaning speed: max fast speed over targets (new; Tricco & Price 2012 sec. 2) */
+    if (cfg->mhd && cfg->cleaning) {
+        FN(prim)* pr = (FN(prim)*)malloc((size_t)n * sizeof(FN(prim)) + 8);
+        FN(prims)(cfg, st, pr);
+        for (int64_t i = 0; i < n; ++i)
+            if (is_target(st->species[i], st->mask[i]) && (double)pr[i].c > chmax) chmax = (double)pr[i].c;
+        free(pr);
+    }
+    st->c_h = chmax;
+    return 0;
+}
+
+/* Force pass: HydroInteraction (HydroForces.hpp:139-175) + Monaghan viscosity + SPMHD + cleaning. */
+int FN(orc_forces)(const sph_run_config* cfg, int32_t prec, orc_state* st, const uint32_t* cell_start)
+{
+    orc_grid g;
+    if (orc_make_grid(cfg, &g)) return -1;
+    int64_t n = st->n;
+    REAL* pc[3];
+    for (int a = 0; a < 3; ++a) pc[a] = (REAL*)malloc((size_t)n * sizeof(REAL) + 8);
+    FN(pair_coords)(&g, prec, st, pc);
+    FN(prim)* pr = (FN(prim)*)malloc((size_t)n * sizeof(FN(prim)) + 8);
+    FN(prims)(cfg, st, pr);
+    const REAL H2 = (prec == ORC_PREC_F64) ? (REAL)g.H2d : (REAL)g.H2;
+    const int kern = cfg->kernel;
+    const int mhd = cfg->mhd, clean = cfg->mhd && cfg->cleaning;
+    const REAL alpha = (REAL)cfg->alpha_visc, beta = (REAL)cfg->beta_visc;
+    const REAL borve = (mhd && cfg->borve) ? (REAL)1 : (REAL)0;
+    const REAL ch = (REAL)st->c_h;
+    const REAL csig = (REAL)cfg->clean_sigma;
+    const REAL* m = RA(st->mass);
+    const REAL* hh = RA(st->h);
+    const REAL* rho = RA(st->rho);
+    const REAL *vx = RA(st->vel[0]), *vy = RA(st->vel[1]), *vz = RA(st->vel[2]);
+    const REAL *Bx = mhd ? RA(st->B[0]) : NULL, *By = mhd ? RA(st->B[1]) : NULL,
+               *Bz = mhd ? RA(st->B[2]) : NULL;
+
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int cell = 0; cell < g.ncell; ++cell) {
+        int c[3];
+        cell_coords(&g, cell, c);
+        int ncc = 0, ccs[27];
+        REAL sh[27][3];
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int d[3] = {dx, dy, dz};
+                    if (FN(nb_cell)(&g, prec, cfg, c, d, &ccs[ncc], sh[ncc])) {
+                        if (dx == 0 && dy == 0 && dz == 0) ccs[ncc] |= 0x80000000u;
+                        ++ncc;
+                    }
+                }
+        for (int64_t i = cell_start[cell]; i < (int64_t)cell_start[cell + 1]; ++i) {
+            if (!is_target(st->species[i], st->mask[i])) continue;
+            const FN(prim) P_i = pr[i];
+            REAL pi[3] = {pc[0][i], pc[1][i], pc[2][i]};
+            REAL bi[3] = {0, 0, 0};
+            if (mhd) { bi[0] = Bx[i]; bi[1] = By[i]; bi[2] = Bz[i]; }
+            REAL ax = 0, ay = 0, az = 0, DV = 0, AVU = 0, vmax = 0;
+            REAL Ti = 0, Tj = 0, TBx = 0, TBy = 0, TBz = 0, VBx = 0, VBy = 0, VBz = 0, DB = 0;
+            REAL PSx = 0, PSy = 0, PSz = 0;
+            double sa = 0, sdu = 0, sdb = 0, sdiv = 0, sdvv = 0;
+            for (int k = 0; k < ncc; ++k) {
+                int own = (ccs[k] & 0x80000000u) != 0;
+                int cc = ccs[k] & 0x7fffffff;
+                for (int64_t j = cell_start[cc]; j < (int64_t)cell_start[cc + 1]; ++j) {
+                    if (!is_source(st->species[j], st->mask[j])) continue;
+                    REAL d[3];
+                    REAL r2 = FN(geom)(pi, pc, j, sh[k], d);
+                    if (!(r2 < H2)) continue;
+                    if (own && j == i) continue;
+                    const FN(prim) P_j = pr[j];
+                    REAL invR = (REAL)0, rr = (REAL)0;
+                    if (r2 > (REAL)0) {
+                        invR = (REAL)1 / (REAL)sqrt(r2);
+                        rr = r2 * invR;
+                    }
+                    REAL s_i = FN(kgrad)(kern, rr, invR, rr * P_i.invh, P_i.gfac);
+                    REAL s_j = FN(kgrad)(kern, rr, invR, rr * P_j.invh, P_j.gfac);
+                    REAL dvx = vx[i] - vx[j], dvy = vy[i] - vy[j], dvz = vz[i] - vz[j];
+                    REAL vr = dvx * d[0] + dvy * d[1] + dvz * d[2];
+                    REAL ms_i = m[j] * s_i, ms_j = m[j] * s_j;
+                    /* pressure (+ magnetic pressure) gradient, symmetric form */
+                    REAL fiso = P_i.pr * ms_i + P_j.pr * ms_j;
+                    REAL cax = -fiso * d[0], cay = -fiso * d[1], caz = -fiso * d[2];
+                    DV += ms_i * vr;
+                    sdvv += fabs((double)(ms_i * vr));
+                    sdu += fabs((double)(P_i.ph * ms_i * vr));
+                    /* signal velocity */
+                    REAL w = vr * invR;
+                    REAL vs = P_i.c + P_j.c - beta * (w < (REAL)0 ? w : (REAL)0);
+                    if (vs > vmax) vmax = vs;
+                    /* artificial viscosity */
+                    if (alpha > (REAL)0 && vr < (REAL)0) {
+                        REAL hb = (REAL)0.5 * (hh[i] + hh[j]);
+                        REAL mu = hb * vr / (r2 + (REAL)0.01 * hb * hb);
+                        REAL cb = (REAL)0.5 * (P_i.c + P_j.c);
+                        REAL rb = (REAL)0.5 * (rho[i] + rho[j]);
+                        REAL Pi = (beta * mu - alpha * cb) * mu / rb;
+                        REAL gv = Pi * (REAL)0.5 * (ms_i + ms_j);
+                        cax -= gv * d[0];
+                        cay -= gv * d[1];
+                        caz -= gv * d[2];
+                        AVU += gv * vr;
+                        sdu += 0.5 * fabs((double)(gv * vr));
+                    }
+                    ax += cax;
+                    ay += cay;
+                    az += caz;
+                    double cnorm = sqrt((double)cax * cax + (double)cay * cay + (double)caz * caz);
+                    if (mhd) {
+                        REAL Bir = bi[0] * d[0] + bi[1] * d[1] + bi[2] * d[2];
+                        REAL Bjr = Bx[j] * d[0] + By[j] * d[1] + Bz[j] * d[2];
+                        REAL ti = Bir * P_i.bdr * ms_i;
+                        REAL tj = Bjr * P_j.bdr * ms_j;
+                        Ti += ti;
+                        Tj += tj;
+                        TBx += tj * Bx[j];
+                        TBy += tj * By[j];
+                        TBz += tj * Bz[j];
+                        REAL wv = ms_i * Bir;
+                        VBx += wv * dvx;
+                        VBy += wv * dvy;
+                        VBz += wv * dvz;
+                        REAL db = ms_i * (Bir - Bjr);
+                        DB += db;
+                        double bn_i = sqrt((double)bi[0] * bi[0] + (double)bi[1] * bi[1] + (double)bi[2] * bi[2]);
+                        double bn_j = sqrt((double)Bx[j] * Bx[j] + (double)By[j] * By[j] + (double)Bz[j] * Bz[j]);
+                        cnorm += fabs((double)ti) * bn_i + fabs((double)tj) * bn_j
+                                 + (double)borve * fabs((double)ti + (double)tj) * bn_i;
+                        double dvn = sqrt((double)dvx * dvx + (double)dvy * dvy + (double)dvz * dvz);
+                        sdb += (fabs((double)wv) * dvn + bn_i * fabs((double)(ms_i * vr))) / (double)rho[i];
+                        sdiv += fabs((double)db) / (double)rho[i];
+                        if (clean) {
+                            REAL fps = P_i.psr * ms_i + P_j.psr * ms_j;
+                            PSx += fps * d[0];
+                            PSy += fps * d[1];
+                            PSz += fps * d[2];
+                            double dn = sqrt((double)d[0] * d[0] + (double)d[1] * d[1] + (double)d[2] * d[2]);
+                            sdb += (double)rho[i] * fabs((double)fps) * dn;
+                        }
+                    }
+                    sa += cnorm;
+                }
+            }
+            REAL invrho = (REAL)1 / rho[i];
+            if (mhd) {
+                REAL tcoef = Ti - borve * (Ti + Tj);
+                ax = ax + TBx + bi[0] * tcoef;
+                ay = ay + TBy + bi[1] * tcoef;
+                az = az + TBz + bi[2] * tcoef;
+            }
+            RA(st->dvdt[0])[i] = ax;
+            RA(st->dvdt[1])[i] = ay;
+            RA(st->dvdt[2])[i] = az;
+            RA(st->dudt)[i] = P_i.ph * DV + (REAL)0.5 * AVU;
+            REAL vsg = (REAL)2 * P_i.c;
+            if (vmax > vsg) vsg = vmax;
+            RA(st->vsig)[i] = vsg;
+            if (mhd) {
+                REAL divB = -DB * invrho;
+                REAL divv = -DV * invrho;
+                REAL dbx = -(VBx - bi[0] * DV) * invrho;
+                REAL dby = -(VBy - bi[1] * DV) * invrho;
+                REAL dbz = -(VBz - bi[2] * DV) * invrho;
+                REAL dpsi = (REAL)0;
+                if (clean) {
+                    dbx -= rho[i] * PSx;
+                    dby -= rho[i] * PSy;
+                    dbz -= rho[i] * PSz;
+                    REAL psi_i = RA(st->psi)[i];
+                    dpsi = -ch * ch * divB - psi_i * csig * ch * P_i.invh - (REAL)0.5 * psi_i * divv;
+                }
+                RA(st->dbdt[0])[i] = dbx;
+                RA(st->dbdt[1])[i] = dby;
+                RA(st->dbdt[2])[i] = dbz;
+                RA(st->dpsidt)[i] = dpsi;
+                RA(st->divb)[i] = divB;
+            }
+            if (st->sc_a) st->sc_a[i] = sa;
+            if (st->sc_du) st->sc_du[i] = sdu;
+            if (st->sc_db) st->sc_db[i] = sdb;
+            if (st->sc_divb) st->sc_divb[i] = sdiv;
+            if (st->sc_dpsi && clean) {
+                double psi_i = (double)RA(st->psi)[i];
+                st->sc_dpsi[i] = (double)ch * (double)ch * sdiv
+                                 + fabs(psi_i * (double)csig * (double)ch * (double)P_i.invh)
+                                 + 0.5 * fabs(psi_i) * sdvv / (double)rho[i];
+            }
+        }
+    }
+    free(pr);
+    for (int a = 0; a < 3; ++a) free(pc[a]);
+    return 0;
+}
+
+/* Drift: PushVelocity (PushVelocity.hpp:35-42) for Movable species, then periodic wrap. */
+int FN(orc_drift)(const sph_run_config* cfg, int32_t prec, orc_state* st, double dt)
+{
+    orc_grid g;
+    if (orc_make_grid(cfg, &g)) return -1;
+    int64_t n = st->n;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        if (!is_movable(st->species[i], st->mask[i])) continue;
+        for (int a = 0; a < 3; ++a) {
+            REAL v = RA(st->vel[a])[i];
+            if (prec == ORC_PREC_PLAIN) {
+                float x = (float)st->pos[a][i];
+                float dts = (float)dt;
+                float vv = (float)v;
+                x = x + vv * dts;
+                if (cfg->periodic[a]) {
+                    float hi = g.lo32[a] + g.L32[a];
+                    if (x >= hi) x = x - g.L32[a];
+                    else if (x < g.lo32[a]) x = x + g.L32[a];
+                }
+                st->pos[a][i] = (double)x;
+            } else {
+                double x = st->pos[a][i];
+                x = x + (double)v * dt;
+                if (cfg->periodic[a]) {
+                    double hi = g.lo[a] + g.L[a];
+                    if (x >= hi) x = x - g.L[a];
+                    else if (x < g.lo[a]) x = x + g.L[a];
+                }
+                st->pos[a][i] = x;
+            }
+        }
+    }
+    return 0;
+}
+
+/* CFL dt of one target (new; Monaghan 1997 signal velocity, Courant + force criteria). */
+static inline double FN(dt_particle)(const sph_run_config* cfg, int32_t prec, REAL h, REAL vsig, REAL ax,
+                                     REAL ay, REAL az)
+{
+    if (prec == ORC_PREC_PLAIN) {
+        float dti = (float)cfg->cfl * (float)h / (float)vsig;
+        float a2 = (float)ax * (float)ax + (float)ay * (float)ay + (float)az * (float)az;
+        if (a2 > 0.0f) {
+            float dtf = (float)cfg->cfl_force * sqrtf((float)h / sqrtf(a2));
+            if (dtf < dti) dti = dtf;
+        }
+        return (double)dti;
+    }
+    double dti = (double)cfg->cfl * (double)h / (double)vsig;
+    double a2 = (double)ax * (double)ax + (double)ay * (double)ay + (double)az * (double)az;
+    if (a2 > 0.0) {
+        double dtf = (double)cfg->cfl_force * sqrt((double)h / sqrt(a2));
+        if (dtf < dti) dti = dtf;
+    }
+    return dti;
+}
+
+/* FP64 reductions over targets: min dt, mass, momentum, energy. */
+int FN(orc_reduce)(const sph_run_config* cfg, int32_t prec, orc_state* st, sph_conserved* out)
+{
+    double M = 0, Px = 0, Py = 0, Pz = 0, E = 0, dtmin = INFINITY;
+    for (int64_t i = 0; i < st->n; ++i) {
+        if (!is_target(st->species[i], st->mask[i])) continue;
+        double m = (double)RA(st->mass)[i];
+        double vx = (double)RA(st->vel[0])[i], vy = (double)RA(st->vel[1])[i], vz = (double)RA(st->vel[2])[i];
+        double v2 = vx * vx + vy * vy + vz * vz;
+        double b2 = 0;
+        if (cfg->mhd) {
+            double bx = (double)RA(st->B[0])[i], by = (double)RA(st->B[1])[i], bz = (double)RA(st->B[2])[i];
+            b2 = bx * bx + by * by + bz * bz;
+        }
+        M += m;
+        Px += m * vx;
+        Py += m * vy;
+        Pz += m * vz;
+        E += m * (0.5 * v2 + (double)RA(st->u)[i] + 0.5 * b2 / (double)RA(st->rho)[i]);
+        double dti = FN(dt_particle)(cfg, prec, RA(st->h)[i], RA(st->vsig)[i], RA(st->dvdt[0])[i],
+                                     RA(st->dvdt[1])[i], RA(st->dvdt[2])[i]);
+        if (dti < dtmin) dtmin = dti;
+    }
+    if (out) {
+        out->mass = M;
+        out->momentum[0] = Px;
+        out->momentum[1] = Py;
+        out->momentum[2] = Pz;
+        out->energy = E;
+        out->dt_next = cfg->dt_fixed > 0 ? cfg->dt_fixed : dtmin;
+        out->time = st->time;
+        out->c_h = st->c_h;
+    }
+    return 0;
+}
+
+/* Kick: EulerIntegrate (EulerIntegrate.hpp:43-52) on Movable+Thermodynamic, extended to B and psi,
+ * then the reductions of the end-of-step state. */
+int FN(orc_kick)(const sph_run_config* cfg, int32_t prec, orc_state* st, double dt, sph_conserved* out)
+{
+    REAL dtr = (REAL)dt;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < st->n; ++i) {
+        if (!is_target(st->species[i], st->mask[i])) continue;
+        for (int a = 0; a < 3; ++a) RA(st->vel[a])[i] = RA(st->vel[a])[i] + RA(st->dvdt[a])[i] * dtr;
+        RA(st->u)[i] = RA(st->u)[i] + RA(st->dudt)[i] * dtr;
+        if (cfg->mhd) {
+            for (int a = 0; a < 3; ++a) RA(st->B[a])[i] = RA(st->B[a])[i] + RA(st->dbdt[a])[i] * dtr;
+            RA(st->psi)[i] = RA(st->psi)[i] + RA(st->dpsidt)[i] * dtr;
+        }
+    }
+    st->time += dt;
+    return FN(orc_reduce)(cfg, prec, st, out);
+}
+
+int FN(orc_prime)(const sph_run_config* cfg, int32_t prec, orc_state* st, sph_conserved* out)
+{
+    orc_grid g;
+    if (orc_make_grid(cfg, &g)) return -1;
+    uint32_t* cs = (uint32_t*)malloc(((size_t)g.ncell + 1) * 4);
+    int rc = FN(orc_resort)(cfg, prec, st, cs);
+    if (!rc) rc = FN(orc_density)(cfg, prec, st, cs);
+    if (!rc) rc = FN(orc_forces)(cfg, prec, st, cs);
+    sph_conserved c;
+    if (!rc) rc = FN(orc_reduce)(cfg, prec, st, &c);
+    if (!rc) st->dt = c.dt_next;
+    if (out) *out = c;
+    free(cs);
+    return rc;
+}
+
+/* Simulation::runOneStep order (Simulation.x.cpp:196-310): drift, resort, density, forces, kick. */
+int FN(orc_step)(const sph_run_config* cfg, int32_t prec, orc_state* st, int32_t n_steps, sph_conserved* out)
+{
+    orc_grid g;
+    if (orc_make_grid(cfg, &g)) return -1;
+    uint32_t* cs = (uint32_t*)malloc(((size_t)g.ncell + 1) * 4);
+    int rc = 0;
+    sph_conserved c;
+    memset(&c, 0, sizeof(c));
+    for (int s = 0; s < n_steps && !rc; ++s) {
+        double dt = cfg->dt_fixed > 0 ? cfg->dt_fixed : st->dt;
+        if (!(dt > 0)) { rc = -4; break; }
+        rc = FN(orc_drift)(cfg, prec, st, dt);
+        if (!rc) rc = FN(orc_resort)(cfg, prec, st, cs);
+        if (!rc) rc = FN(orc_density)(cfg, prec, st, cs);
+        if (!rc) rc = FN(orc_forces)(cfg, prec, st, cs);
+        if (!rc) rc = FN(orc_kick)(cfg, prec, st, dt, &c);
+        if (!rc) st->dt = c.dt_next;
+    }
+    if (out) *out = c;
+    free(cs);
+    return rc;
+}
+
+/* Brute-force O(N^2) neighbour counts over all periodic images (absolute coordinates only):
+ * the check on the supercell stencil. */
+int FN(orc_brute_counts)(const sph_run_config* cfg, int32_t prec, orc_state* st)
+{
+    if (prec == ORC_PREC_MIXED) return -1;
+    orc_grid g;
+    if (orc_make_grid(cfg, &g)) return -1;
+    int64_t n = st->n;
+    REAL* pc[3];
+    for (int a = 0; a < 3; ++a) pc[a] = (REAL*)malloc((size_t)n * sizeof(REAL) + 8);
+    FN(pair_coords)(&g, prec, st, pc);
+    const REAL H2 = (prec == ORC_PREC_F64) ? (REAL)g.H2d : (REAL)g.H2;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+        if (!is_target(st->species[i], st->mask[i])) { st->nnb[i] = 0; continue; }
+        uint32_t cnt = 0;
+        REAL pi[3] = {pc[0][i], pc[1][i], pc[2][i]};
+        for (int64_t j = 0; j < n; ++j) {
+            if (!is_source(st->species[j], st->mask[j])) continue;
+            for (int wz = -1; wz <= 1; ++wz)
+                for (int wy = -1; wy <= 1; ++wy)
+                    for (int wx = -1; wx <= 1; ++wx) {
+                        int w[3] = {wx, wy, wz};
+                        REAL sh[3];
+                        int ok = 1;
+                        for (int a = 0; a < 3; ++a) {
+                            if (!cfg->periodic[a] && w[a] != 0) ok = 0;
+                            sh[a] = (prec == ORC_PREC_PLAIN) ? (REAL)w[a] * (REAL)g.L32[a]
+                                                             : (REAL)((double)w[a] * g.L[a]);
+                        }
+                        if (!ok) continue;
+                        if (j == i && wx == 0 && wy == 0 && wz == 0) continue;
+                        REAL d[3];
+                        REAL r2 = FN(geom)(pi, pc, j, sh, d);
+                        if (r2 < H2) ++cnt;
+                    }
+        }
+        st->nnb[i] = cnt;
+    }
+    for (int a = 0; a < 3; ++a) free(pc[a]);
+    return 0;
+}
+
+#undef FN
+#undef RA

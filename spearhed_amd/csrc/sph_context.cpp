@@ -1,0 +1,890 @@
+// sph_context.cpp -- host runtime behind the C ABI (include/spearhed_sph.h): device memory layout,
+// the stepper that reproduces Simulation::runOneStep's stage order
+// (include/spearhed/control/Simulation.x.cpp:196-310), hipGraph capture and the autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spearhed_sph.h"
+#include "sph_device.hpp"
+#include "sph_launch.hpp"
+
+using namespace sph;
+
+struct sph_ctx {
+    std::string err;
+    int prec = SPH_PREC_MIXED;
+    bool mixed = true;
+    bool has_mhd = false;
+    int nruns = 0;
+    std::vector<sph_run_config> cfg;
+    std::vector<RunDev> runs_h;
+    RunDev* runs_d = nullptr;
+    int64_t ntot = 0;
+    int ncell_tot = 0;
+    int ncell_cap = 0;
+    int nkb = 0;
+    StateBuf S[2] = {};
+    int cur = 0;
+    WorkBuf W = {};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    sph_tuning tune = {256, 128, 0, 16};
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
+    bool binned = false;   // keys + histogram valid for the current positions
+    bool sorted = false;   // buffers sorted by (cell, id) for the current positions
+    bool dens_valid = false;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_parity = -1;
+    // profiling: (SPH_NUM_STAGES + 1) events per recorded step
+    std::vector<hipEvent_t> prof_ev;
+    int prof_cap = 0;
+    int prof_steps = 0;
+    bool profiling = false;
+};
+
+namespace {
+
+int fail(sph_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPTRY(ctx, expr)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail((ctx), SPH_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int support_of(int kernel) { return kernel == SPH_KERNEL_QUINTIC ? 3 : 2; }
+
+float f0_of(int kernel)
+{
+    if (kernel == SPH_KERNEL_QUINTIC) return 66.0f;  // 3^5 - 6*2^5 + 15
+    return 1.0f;
+}
+
+float sigma_of(int kernel)
+{
+    const float pi = 3.14159265358979323846f;
+    if (kernel == SPH_KERNEL_CUBIC) return 1.0f / pi;
+    if (kernel == SPH_KERNEL_QUINTIC) return 1.0f / (120.0f * pi);
+    return 21.0f / (16.0f * pi);
+}
+
+// Supercell grid of one run: per axis the largest count whose edge is >= cell_factor*H*(1+1e-4).
+bool make_run(const sph_run_config& c, RunDev& R, std::string& why)
+{
+    memset(&R, 0, sizeof(R));
+    if (!(c.h0 > 0.0f)) { why = "h0 must be > 0"; return false; }
+    if (c.kernel < 0 || c.kernel > 2) { why = "unknown kernel id"; return false; }
+    if (!(c.gamma > 1.0f)) { why = "gamma must be > 1"; return false; }
+    R.support = support_of(c.kernel);
+    R.H = (float)R.support * c.h0;
+    R.H2 = R.H * R.H;
+    const double f = c.cell_factor > 1.0f ? (double)c.cell_factor : 1.0;
+    int64_t nc = 1;
+    for (int a = 0; a < 3; ++a) {
+        const double L = c.box_len[a];
+        if (!(L > 0)) { why = "box_len must be > 0"; return false; }
+        if (c.periodic[a] && L < (double)R.H * (1.0 + 1e-4)) {
+            why = "periodic box edge shorter than the interaction radius";
+            return false;
+        }
+        int n = (int)std::floor(L / ((double)R.H * f * (1.0 + 1e-4)));
+        if (n < 1) n = 1;
+        R.n[a] = n;
+        R.periodic[a] = c.periodic[a] ? 1 : 0;
+        R.lo[a] = c.box_lo[a];
+        R.L[a] = L;
+        R.s[a] = L / (double)n;
+        R.inv_s[a] = (double)n / L;
+        R.lo32[a] = (float)R.lo[a];
+        R.L32[a] = (float)L;
+        R.s32[a] = (float)R.s[a];
+        R.inv_s32[a] = (float)R.inv_s[a];
+        nc *= n;
+    }
+    if (nc > (1LL << 30)) { why = "too many supercells"; return false; }
+    R.ncell = (int32_t)nc;
+    R.kernel = c.kernel;
+    R.mhd = c.mhd ? 1 : 0;
+    R.cleaning = (c.mhd && c.cleaning) ? 1 : 0;
+    R.gamma = c.gamma;
+    R.alpha = c.alpha_visc;
+    R.beta = c.beta_visc;
+    R.borve = (c.mhd && c.borve) ? 1.0f : 0.0f;
+    R.clean_sigma = c.clean_sigma;
+    R.cfl = c.cfl;
+    R.cfl_force = c.cfl_force;
+    R.sigma = sigma_of(c.kernel);
+    R.f0 = f0_of(c.kernel);
+    R.dt_fixed = c.dt_fixed;
+    return true;
+}
+
+struct Carver {
+    char* base;
+    size_t off = 0;
+    bool dry;
+    template <class T>
+    T* take(int64_t count)
+    {
+        off = (off + 255) & ~size_t(255);
+        T* p = dry ? nullptr : reinterpret_cast<T*>(base + off);
+        off += sizeof(T) * (size_t)std::max<int64_t>(count, 1);
+        return p;
+    }
+};
+
+void carve(Carver& cv, sph_ctx* c, int64_t N, int ncell_cap, int nkb, int nruns)
+{
+    for (int b = 0; b < 2; ++b) {
+        StateBuf& S = c->S[b];
+        S.id = cv.take<uint64_t>(N);
+        S.mask = cv.take<uint8_t>(N);
+        S.species = cv.take<uint8_t>(N);
+        for (int a = 0; a < 3; ++a) {
+            if (c->mixed) { S.pos64[a] = cv.take<double>(N); S.pos32[a] = nullptr; }
+            else { S.pos32[a] = cv.take<float>(N); S.pos64[a] = nullptr; }
+            S.vel[a] = cv.take<float>(N);
+            S.B[a] = c->has_mhd ? cv.take<float>(N) : nullptr;
+        }
+        S.mass = cv.take<float>(N);
+        S.h = cv.take<float>(N);
+        S.u = cv.take<float>(N);
+        S.rho = cv.take<float>(N);
+        S.psi = c->has_mhd ? cv.take<float>(N) : nullptr;
+    }
+    WorkBuf& W = c->W;
+    for (int a = 0; a < 3; ++a) {
+        W.dvdt[a] = cv.take<float>(N);
+        W.dbdt[a] = c->has_mhd ? cv.take<float>(N) : nullptr;
+    }
+    W.dudt = cv.take<float>(N);
+    W.dpsidt = c->has_mhd ? cv.take<float>(N) : nullptr;
+    W.divb = c->has_mhd ? cv.take<float>(N) : nullptr;
+    W.vsig = cv.take<float>(N);
+    W.nnb = cv.take<uint32_t>(N);
+    W.key = cv.take<uint32_t>(N);
+    W.rank = cv.take<uint32_t>(N);
+    W.tmp = cv.take<uint32_t>(N);
+    W.count = cv.take<uint32_t>(ncell_cap);
+    W.cs = cv.take<uint32_t>((int64_t)ncell_cap + 1);
+    W.bsum = cv.take<uint32_t>((ncell_cap + 2047) / 2048 + 1);
+    W.P0 = cv.take<float4>(N);
+    W.P1 = cv.take<float4>(N);
+    W.P2 = cv.take<float4>(N);
+    W.P3 = cv.take<float4>(N);
+    W.P4 = c->has_mhd ? cv.take<float4>(N) : nullptr;
+    W.dt = cv.take<double>(nruns);
+    W.time = cv.take<double>(nruns);
+    W.chbits = cv.take<uint32_t>(nruns);
+    W.partial = cv.take<double>((int64_t)nkb * 6);
+    W.cons = cv.take<double>((int64_t)nruns * 8);
+    W.blk_run = cv.take<int32_t>(nkb);
+}
+
+LaunchCtx lctx(sph_ctx* c)
+{
+    LaunchCtx L;
+    L.runs = c->runs_d;
+    L.nruns = c->nruns;
+    L.ncell_tot = c->ncell_tot;
+    L.nkb = c->nkb;
+    L.mixed = c->mixed ? 1 : 0;
+    L.has_mhd = c->has_mhd ? 1 : 0;
+    L.density_chunk = c->tune.density_chunk;
+    L.force_chunk = c->tune.force_chunk;
+    int k = std::max(1, std::min(64, c->tune.max_lanes_per_target));
+    int lk = 0;
+    while ((2 << lk) <= k) ++lk;
+    L.kmax_log = lk;
+    L.S = c->S[c->cur];
+    L.W = c->W;
+    L.stream = c->stream;
+    return L;
+}
+
+void invalidate_graph(sph_ctx* c)
+{
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->graph_parity = -1;
+}
+
+int do_bin(sph_ctx* c, double dt, int drift)
+{
+    if (c->binned) HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_tot, c->stream));
+    LaunchCtx L = lctx(c);
+    HIPTRY(c, launch_drift_bin(L, dt, drift));
+    c->binned = true;
+    c->sorted = false;
+    c->dens_valid = false;
+    return SPH_OK;
+}
+
+int do_resort(sph_ctx* c)
+{
+    if (!c->binned) {
+        int rc = do_bin(c, 0.0, 0);
+        if (rc) return rc;
+    }
+    LaunchCtx L = lctx(c);
+    HIPTRY(c, launch_scan(L));
+    HIPTRY(c, launch_scatter_gather(L, c->S[c->cur ^ 1]));
+    c->cur ^= 1;
+    c->binned = false;
+    c->sorted = true;
+    c->dens_valid = false;
+    return SPH_OK;
+}
+
+int do_density(sph_ctx* c)
+{
+    if (!c->sorted) {
+        int rc = do_resort(c);
+        if (rc) return rc;
+    }
+    LaunchCtx L = lctx(c);
+    HIPTRY(c, launch_density(L));
+    c->dens_valid = true;
+    return SPH_OK;
+}
+
+int do_forces(sph_ctx* c)
+{
+    if (!c->dens_valid) return fail(c, SPH_E_STATE, "force pass needs a density pass after the last resort");
+    LaunchCtx L = lctx(c);
+    HIPTRY(c, launch_forces(L));
+    return SPH_OK;
+}
+
+int do_kick(sph_ctx* c, double dt, int kick)
+{
+    LaunchCtx L = lctx(c);
+    HIPTRY(c, launch_kick(L, dt, kick));
+    return SPH_OK;
+}
+
+int prof_mark(sph_ctx* c, int k)
+{
+    if (!c->profiling || c->prof_steps >= c->prof_cap) return SPH_OK;
+    HIPTRY(c, hipEventRecord(c->prof_ev[(size_t)c->prof_steps * (SPH_NUM_STAGES + 1) + k], c->stream));
+    return SPH_OK;
+}
+
+// One whole step in runOneStep order: drift (+bin), resort, density, forces, kick (+reductions).
+int do_step(sph_ctx* c)
+{
+    int rc = prof_mark(c, 0);
+    if (!rc) rc = do_bin(c, 0.0, 1);
+    if (!rc) rc = prof_mark(c, 1);
+    if (!rc) rc = do_resort(c);
+    if (!rc) rc = prof_mark(c, 2);
+    if (!rc) rc = do_density(c);
+    if (!rc) rc = prof_mark(c, 3);
+    if (!rc) rc = do_forces(c);
+    if (!rc) rc = prof_mark(c, 4);
+    if (!rc) rc = do_kick(c, 0.0, 1);
+    if (!rc) rc = prof_mark(c, 5);
+    if (!rc && c->profiling && c->prof_steps < c->prof_cap) ++c->prof_steps;
+    return rc;
+}
+
+int alloc_ctx(sph_ctx* c, const int64_t* capacity)
+{
+    c->runs_h.resize(c->nruns);
+    int64_t p0 = 0;
+    int c0 = 0, kb = 0;
+    int64_t ncell_max_total = 0;
+    for (int r = 0; r < c->nruns; ++r) {
+        std::string why;
+        if (!make_run(c->cfg[r], c->runs_h[r], why)) return fail(c, SPH_E_INVALID, "run " + std::to_string(r) + ": " + why);
+        if (capacity[r] < 1) return fail(c, SPH_E_INVALID, "run capacity must be >= 1");
+        RunDev& R = c->runs_h[r];
+        R.p0 = p0;
+        R.np = capacity[r];
+        R.c0 = c0;
+        R.kb_first = kb;
+        R.kb_count = (int32_t)((R.np + 255) / 256);
+        p0 += R.np;
+        c0 += R.ncell;
+        kb += R.kb_count;
+        // capacity for the autotuner's finest grid (cell_factor 1)
+        sph_run_config fine = c->cfg[r];
+        fine.cell_factor = 1.0f;
+        RunDev Rf;
+        make_run(fine, Rf, why);
+        ncell_max_total += std::max(Rf.ncell, R.ncell);
+    }
+    if (p0 >= (1LL << 32) - 1) return fail(c, SPH_E_CAPACITY, "more than 2^32-1 particles in one context");
+    if (ncell_max_total >= (1LL << 31)) return fail(c, SPH_E_CAPACITY, "too many supercells");
+    c->ntot = p0;
+    c->ncell_tot = c0;
+    c->ncell_cap = (int)ncell_max_total;
+    c->nkb = kb;
+    Carver dry{nullptr, 0, true};
+    carve(dry, c, c->ntot, c->ncell_cap, c->nkb, c->nruns);
+    c->arena_bytes = dry.off + 256;
+    HIPTRY(c, hipMalloc(&c->arena, c->arena_bytes));
+    Carver real{reinterpret_cast<char*>(c->arena), 0, false};
+    carve(real, c, c->ntot, c->ncell_cap, c->nkb, c->nruns);
+    // zero the whole arena first (histogram, accumulators), then write the tables into it
+    HIPTRY(c, hipMemset(c->arena, 0, c->arena_bytes));
+    HIPTRY(c, hipMalloc(&c->runs_d, sizeof(RunDev) * c->nruns));
+    HIPTRY(c, hipMemcpy(c->runs_d, c->runs_h.data(), sizeof(RunDev) * c->nruns, hipMemcpyHostToDevice));
+    std::vector<int32_t> blk(c->nkb);
+    for (int r = 0; r < c->nruns; ++r)
+        for (int b = 0; b < c->runs_h[r].kb_count; ++b) blk[c->runs_h[r].kb_first + b] = r;
+    HIPTRY(c, hipMemcpy(c->W.blk_run, blk.data(), sizeof(int32_t) * c->nkb, hipMemcpyHostToDevice));
+    std::vector<double> dt0(c->nruns);
+    for (int r = 0; r < c->nruns; ++r) dt0[r] = c->cfg[r].dt_fixed > 0 ? c->cfg[r].dt_fixed : 0.0;
+    HIPTRY(c, hipMemcpy(c->W.dt, dt0.data(), sizeof(double) * c->nruns, hipMemcpyHostToDevice));
+    return SPH_OK;
+}
+
+// Re-derive every run's grid (after a cell_factor change) keeping particle offsets.
+int regrid(sph_ctx* c)
+{
+    int c0 = 0;
+    for (int r = 0; r < c->nruns; ++r) {
+        RunDev R;
+        std::string why;
+        if (!make_run(c->cfg[r], R, why)) return fail(c, SPH_E_INVALID, why);
+        R.p0 = c->runs_h[r].p0;
+        R.np = c->runs_h[r].np;
+        R.kb_first = c->runs_h[r].kb_first;
+        R.kb_count = c->runs_h[r].kb_count;
+        R.c0 = c0;
+        c0 += R.ncell;
+        c->runs_h[r] = R;
+    }
+    if (c0 > c->ncell_cap) return fail(c, SPH_E_CAPACITY, "supercell table capacity exceeded");
+    c->ncell_tot = c0;
+    HIPTRY(c, hipMemcpyAsync(c->runs_d, c->runs_h.data(), sizeof(RunDev) * c->nruns, hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_cap, c->stream));
+    c->binned = false;
+    c->sorted = false;
+    c->dens_valid = false;
+    invalidate_graph(c);
+    return SPH_OK;
+}
+
+template <class T>
+int copy_col(sph_ctx* c, T* dst_dev, const T* src_host, int64_t n, T fill)
+{
+    if (src_host) {
+        HIPTRY(c, hipMemcpyAsync(dst_dev, src_host, sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
+    } else {
+        std::vector<T> z(n, fill);
+        HIPTRY(c, hipMemcpyAsync(dst_dev, z.data(), sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
+        HIPTRY(c, hipStreamSynchronize(c->stream));
+    }
+    return SPH_OK;
+}
+
+template <class T>
+int fetch_col(sph_ctx* c, T* dst_host, const T* src_dev, int64_t n)
+{
+    if (!dst_host || !src_dev) return SPH_OK;
+    HIPTRY(c, hipMemcpyAsync(dst_host, src_dev, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+    return SPH_OK;
+}
+
+}  // namespace
+
+// =================================================================================================
+// C ABI
+// =================================================================================================
+extern "C" {
+
+int sph_ctx_create(const sph_run_config* runs, const int64_t* capacity, int32_t n_runs, int32_t precision,
+                   void* stream, sph_ctx** out)
+{
+    if (!out || !runs || !capacity || n_runs < 1) return SPH_E_INVALID;
+    *out = nullptr;
+    if (precision != SPH_PREC_FP32 && precision != SPH_PREC_MIXED) return SPH_E_INVALID;
+    sph_ctx* c = new sph_ctx();
+    c->prec = precision;
+    c->mixed = precision == SPH_PREC_MIXED;
+    c->nruns = n_runs;
+    c->cfg.assign(runs, runs + n_runs);
+    for (auto& r : c->cfg) c->has_mhd = c->has_mhd || r.mhd;
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return SPH_E_DEVICE;
+        }
+        c->own_stream = true;
+    }
+    int rc = alloc_ctx(c, capacity);
+    *out = c;  // returned even on failure so that sph_last_error can report; caller destroys
+    return rc;
+}
+
+int sph_ctx_destroy(sph_ctx* c)
+{
+    if (!c) return SPH_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    invalidate_graph(c);
+    for (auto e : c->prof_ev) hipEventDestroy(e);
+    if (c->arena) hipFree(c->arena);
+    if (c->runs_d) hipFree(c->runs_d);
+    if (c->own_stream) hipStreamDestroy(c->stream);
+    delete c;
+    return SPH_OK;
+}
+
+const char* sph_last_error(const sph_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int sph_sync(sph_ctx* c)
+{
+    if (!c) return SPH_E_INVALID;
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    return SPH_OK;
+}
+
+int sph_get_grid(sph_ctx* c, int32_t run, sph_grid_info* o)
+{
+    if (!c || !o || run < 0 || run >= c->nruns) return SPH_E_INVALID;
+    const RunDev& R = c->runs_h[run];
+    for (int a = 0; a < 3; ++a) {
+        o->n[a] = R.n[a];
+        o->cell[a] = R.s[a];
+    }
+    o->n_cells = R.ncell;
+    o->radius = R.H;
+    o->particle_offset = R.p0;
+    o->cell_offset = R.c0;
+    return SPH_OK;
+}
+
+int sph_set_tuning(sph_ctx* c, const sph_tuning* t)
+{
+    if (!c || !t) return SPH_E_INVALID;
+    const int dc = t->density_chunk, fc = t->force_chunk;
+    if (dc != 128 && dc != 256 && dc != 512) return fail(c, SPH_E_INVALID, "density_chunk must be 128, 256 or 512");
+    if (fc != 64 && fc != 128 && fc != 256) return fail(c, SPH_E_INVALID, "force_chunk must be 64, 128 or 256");
+    if (t->max_lanes_per_target < 1 || t->max_lanes_per_target > 64)
+        return fail(c, SPH_E_INVALID, "max_lanes_per_target must be in [1, 64]");
+    c->tune = *t;
+    invalidate_graph(c);
+    return SPH_OK;
+}
+
+int sph_get_tuning(sph_ctx* c, sph_tuning* o)
+{
+    if (!c || !o) return SPH_E_INVALID;
+    *o = c->tune;
+    return SPH_OK;
+}
+
+int64_t sph_bytes_per_run(const sph_run_config* cfg, int64_t n, int32_t precision)
+{
+    if (!cfg || n < 0) return -1;
+    RunDev R;
+    std::string why;
+    sph_run_config fine = *cfg;
+    fine.cell_factor = 1.0f;
+    if (!make_run(fine, R, why)) return -1;
+    const int64_t pos = precision == SPH_PREC_MIXED ? 24 : 12;
+    const int64_t mhd = cfg->mhd ? 1 : 0;
+    const int64_t state = 8 + 2 + pos + 12 + 16 + mhd * 16;
+    const int64_t work = 12 + 4 + 4 + 4 + 12 + 64 + mhd * (12 + 4 + 4 + 16);
+    return n * (2 * state + work) + (int64_t)R.ncell * 12 + 4096;
+}
+
+int sph_upload(sph_ctx* c, int32_t run, const sph_host_soa* s)
+{
+    if (!c || !s || run < 0 || run >= c->nruns) return SPH_E_INVALID;
+    const RunDev& R = c->runs_h[run];
+    if (s->n != R.np) return fail(c, SPH_E_INVALID, "upload: n must equal the run's capacity");
+    if (!s->id || !s->pos[0] || !s->pos[1] || !s->pos[2] || !s->mass || !s->vel[0] || !s->vel[1] || !s->vel[2] ||
+        !s->smoothing_length || !s->internal_energy)
+        return fail(c, SPH_E_INVALID, "upload: id, pos, mass, vel, smoothing_length, internal_energy are required");
+    {
+        std::vector<uint64_t> ids(s->id, s->id + s->n);
+        std::sort(ids.begin(), ids.end());
+        if (std::adjacent_find(ids.begin(), ids.end()) != ids.end())
+            return fail(c, SPH_E_INVALID, "upload: particle ids must be unique within a run");
+    }
+    for (int64_t i = 0; i < s->n; ++i)
+        if (!(s->smoothing_length[i] > 0.0f)) return fail(c, SPH_E_INVALID, "upload: smoothing_length must be > 0");
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    const int64_t n = s->n, o = R.p0;
+    StateBuf& S = c->S[c->cur];
+    int rc = 0;
+    rc |= copy_col<uint64_t>(c, S.id + o, s->id, n, 0);
+    rc |= copy_col<uint8_t>(c, S.mask + o, s->mask, n, 1);
+    rc |= copy_col<uint8_t>(c, S.species + o, s->species, n, 0);
+    std::vector<float> tmp;
+    for (int a = 0; a < 3; ++a) {
+        if (c->mixed) {
+            rc |= copy_col<double>(c, S.pos64[a] + o, s->pos[a], n, 0.0);
+        } else {
+            tmp.assign(n, 0.0f);
+            for (int64_t i = 0; i < n; ++i) tmp[i] = (float)s->pos[a][i];
+            rc |= copy_col<float>(c, S.pos32[a] + o, tmp.data(), n, 0.0f);
+            HIPTRY(c, hipStreamSynchronize(c->stream));
+        }
+        rc |= copy_col<float>(c, S.vel[a] + o, s->vel[a], n, 0.0f);
+        if (c->has_mhd) rc |= copy_col<float>(c, S.B[a] + o, s->bfield[a], n, 0.0f);
+    }
+    rc |= copy_col<float>(c, S.mass + o, s->mass, n, 0.0f);
+    rc |= copy_col<float>(c, S.h + o, s->smoothing_length, n, 0.0f);
+    rc |= copy_col<float>(c, S.u + o, s->internal_energy, n, 0.0f);
+    rc |= copy_col<float>(c, S.rho + o, s->density, n, 1.0f);
+    if (c->has_mhd) rc |= copy_col<float>(c, S.psi + o, s->psi, n, 0.0f);
+    if (rc) return SPH_E_DEVICE;
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    if (c->binned) HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_tot, c->stream));
+    c->binned = false;
+    c->sorted = false;
+    c->dens_valid = false;
+    return SPH_OK;
+}
+
+int sph_download(sph_ctx* c, int32_t run, sph_host_soa* d)
+{
+    if (!c || !d || run < 0 || run >= c->nruns) return SPH_E_INVALID;
+    const RunDev& R = c->runs_h[run];
+    if (d->n != R.np) return fail(c, SPH_E_INVALID, "download: n must equal the run's capacity");
+    const int64_t n = R.np, o = R.p0;
+    const StateBuf& S = c->S[c->cur];
+    int rc = 0;
+    rc |= fetch_col(c, d->id, S.id + o, n);
+    rc |= fetch_col(c, d->mask, S.mask + o, n);
+    rc |= fetch_col(c, d->species, S.species + o, n);
+    std::vector<float> tmp[3];
+    for (int a = 0; a < 3; ++a) {
+        if (c->mixed) rc |= fetch_col(c, d->pos[a], S.pos64[a] + o, n);
+        else if (d->pos[a]) { tmp[a].resize(n); rc |= fetch_col(c, tmp[a].data(), S.pos32[a] + o, n); }
+        rc |= fetch_col(c, d->vel[a], S.vel[a] + o, n);
+        rc |= fetch_col(c, d->dvdt[a], c->W.dvdt[a] + o, n);
+        if (c->has_mhd) {
+            rc |= fetch_col(c, d->bfield[a], S.B[a] + o, n);
+            rc |= fetch_col(c, d->dbdt[a], c->W.dbdt[a] + o, n);
+        }
+    }
+    rc |= fetch_col(c, d->mass, S.mass + o, n);
+    rc |= fetch_col(c, d->density, S.rho + o, n);
+    rc |= fetch_col(c, d->smoothing_length, S.h + o, n);
+    rc |= fetch_col(c, d->internal_energy, S.u + o, n);
+    rc |= fetch_col(c, d->dudt, c->W.dudt + o, n);
+    rc |= fetch_col(c, d->vsig, c->W.vsig + o, n);
+    rc |= fetch_col(c, d->n_neighbours, c->W.nnb + o, n);
+    if (c->has_mhd) {
+        rc |= fetch_col(c, d->psi, S.psi + o, n);
+        rc |= fetch_col(c, d->dpsidt, c->W.dpsidt + o, n);
+        rc |= fetch_col(c, d->divb, c->W.divb + o, n);
+    }
+    if (rc) return SPH_E_DEVICE;
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    if (!c->mixed)
+        for (int a = 0; a < 3; ++a)
+            if (d->pos[a])
+                for (int64_t i = 0; i < n; ++i) d->pos[a][i] = (double)tmp[a][i];
+    return SPH_OK;
+}
+
+int sph_stage_drift(sph_ctx* c, double dt)
+{
+    if (!c) return SPH_E_INVALID;
+    return do_bin(c, dt, 1);
+}
+
+int sph_stage_resort(sph_ctx* c)
+{
+    if (!c) return SPH_E_INVALID;
+    return do_resort(c);
+}
+
+int sph_stage_density(sph_ctx* c)
+{
+    if (!c) return SPH_E_INVALID;
+    return do_density(c);
+}
+
+int sph_stage_neighbour_counts(sph_ctx* c)
+{
+    if (!c) return SPH_E_INVALID;
+    return do_density(c);
+}
+
+int sph_stage_forces(sph_ctx* c)
+{
+    if (!c) return SPH_E_INVALID;
+    return do_forces(c);
+}
+
+int sph_stage_integrate(sph_ctx* c, double dt)
+{
+    if (!c) return SPH_E_INVALID;
+    return do_kick(c, dt, 1);
+}
+
+static int read_cons(sph_ctx* c, std::vector<double>& h)
+{
+    h.resize((size_t)c->nruns * 8);
+    HIPTRY(c, hipMemcpyAsync(h.data(), c->W.cons, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    return SPH_OK;
+}
+
+int sph_reduce_min_dt(sph_ctx* c, double* dt)
+{
+    if (!c || !dt) return SPH_E_INVALID;
+    int rc = do_kick(c, 0.0, 0);
+    std::vector<double> h;
+    if (!rc) rc = read_cons(c, h);
+    if (rc) return rc;
+    for (int r = 0; r < c->nruns; ++r) dt[r] = h[(size_t)r * 8 + 5];
+    return SPH_OK;
+}
+
+int sph_reduce_conserved(sph_ctx* c, sph_conserved* o)
+{
+    if (!c || !o) return SPH_E_INVALID;
+    int rc = do_kick(c, 0.0, 0);
+    std::vector<double> h;
+    if (!rc) rc = read_cons(c, h);
+    if (rc) return rc;
+    for (int r = 0; r < c->nruns; ++r) {
+        const double* p = &h[(size_t)r * 8];
+        o[r].mass = p[0];
+        o[r].momentum[0] = p[1];
+        o[r].momentum[1] = p[2];
+        o[r].momentum[2] = p[3];
+        o[r].energy = p[4];
+        o[r].dt_next = p[5];
+        o[r].time = p[6];
+        o[r].c_h = p[7];
+    }
+    return SPH_OK;
+}
+
+int sph_prime(sph_ctx* c)
+{
+    if (!c) return SPH_E_INVALID;
+    int rc = do_resort(c);
+    if (!rc) rc = do_density(c);
+    if (!rc) rc = do_forces(c);
+    if (!rc) rc = do_kick(c, 0.0, 0);
+    return rc;
+}
+
+int sph_step(sph_ctx* c, int32_t n_steps)
+{
+    if (!c || n_steps < 0) return SPH_E_INVALID;
+    for (int s = 0; s < n_steps; ++s) {
+        int rc = do_step(c);
+        if (rc) return rc;
+    }
+    return SPH_OK;
+}
+
+int sph_graph_capture(sph_ctx* c)
+{
+    if (!c) return SPH_E_INVALID;
+    invalidate_graph(c);
+    // one step swaps the buffer sets, so a two-step graph returns to the starting parity
+    const int parity = c->cur;
+    const bool b0 = c->binned, s0 = c->sorted, d0 = c->dens_valid;
+    c->binned = false;  // counts are zero at a step boundary
+    HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = do_step(c);
+    if (!rc) rc = do_step(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->cur = parity;
+    c->binned = b0;
+    c->sorted = s0;
+    c->dens_valid = d0;
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(c, SPH_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    c->graph = g;
+    HIPTRY(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    c->graph_parity = parity;
+    return SPH_OK;
+}
+
+int sph_graph_launch(sph_ctx* c, int32_t n_steps)
+{
+    if (!c || n_steps < 0) return SPH_E_INVALID;
+    if (!c->gexec) return fail(c, SPH_E_STATE, "no captured graph");
+    if (c->binned) {
+        HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_tot, c->stream));
+        c->binned = false;
+    }
+    int s = 0;
+    if (c->cur != c->graph_parity && n_steps > 0) {
+        int rc = do_step(c);
+        if (rc) return rc;
+        ++s;
+    }
+    for (; s + 2 <= n_steps; s += 2) HIPTRY(c, hipGraphLaunch(c->gexec, c->stream));
+    c->sorted = true;
+    c->dens_valid = true;
+    c->binned = false;
+    if (s < n_steps) return do_step(c);
+    return SPH_OK;
+}
+
+int sph_get_sort(sph_ctx* c, int32_t run, uint32_t* key, uint32_t* cell_start)
+{
+    if (!c || run < 0 || run >= c->nruns) return SPH_E_INVALID;
+    if (!c->sorted) return fail(c, SPH_E_STATE, "no resort since the last drift/upload");
+    const RunDev& R = c->runs_h[run];
+    std::vector<uint32_t> cs((size_t)R.ncell + 1);
+    HIPTRY(c, hipMemcpyAsync(cs.data(), c->W.cs + R.c0, sizeof(uint32_t) * cs.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    for (auto& v : cs) v -= (uint32_t)R.p0;
+    if (cell_start) std::copy(cs.begin(), cs.end(), cell_start);
+    if (key)
+        for (int cc = 0; cc < R.ncell; ++cc)
+            for (uint32_t i = cs[cc]; i < cs[cc + 1]; ++i) key[i] = (uint32_t)cc;
+    return SPH_OK;
+}
+
+int sph_profile_begin(sph_ctx* c, int32_t max_steps)
+{
+    if (!c || max_steps < 1) return SPH_E_INVALID;
+    for (auto e : c->prof_ev) hipEventDestroy(e);
+    c->prof_ev.assign((size_t)max_steps * (SPH_NUM_STAGES + 1), nullptr);
+    for (auto& e : c->prof_ev) HIPTRY(c, hipEventCreate(&e));
+    c->prof_cap = max_steps;
+    c->prof_steps = 0;
+    c->profiling = true;
+    return SPH_OK;
+}
+
+int sph_profile_end(sph_ctx* c, sph_profile* out, double* step_ms)
+{
+    if (!c || !out) return SPH_E_INVALID;
+    if (!c->profiling) return fail(c, SPH_E_STATE, "profiling was not started");
+    c->profiling = false;
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    memset(out, 0, sizeof(*out));
+    const int n = c->prof_steps;
+    out->steps = n;
+    std::vector<double> tot(n);
+    for (int s = 0; s < n; ++s) {
+        hipEvent_t* e = &c->prof_ev[(size_t)s * (SPH_NUM_STAGES + 1)];
+        for (int k = 0; k < SPH_NUM_STAGES; ++k) {
+            float ms = 0;
+            HIPTRY(c, hipEventElapsedTime(&ms, e[k], e[k + 1]));
+            out->stage_ms[k] += ms;
+        }
+        float ms = 0;
+        HIPTRY(c, hipEventElapsedTime(&ms, e[0], e[SPH_NUM_STAGES]));
+        tot[s] = ms;
+        if (step_ms) step_ms[s] = ms;
+    }
+    if (n > 0) {
+        for (int k = 0; k < SPH_NUM_STAGES; ++k) out->stage_ms[k] /= n;
+        double sum = 0;
+        for (double v : tot) sum += v;
+        out->step_ms_mean = sum / n;
+        std::vector<double> srt = tot;
+        std::sort(srt.begin(), srt.end());
+        auto pct = [&](double p) {
+            double idx = p * (n - 1);
+            int lo = (int)std::floor(idx), hi = std::min(n - 1, lo + 1);
+            return srt[lo] + (srt[hi] - srt[lo]) * (idx - lo);
+        };
+        out->step_ms_p50 = pct(0.50);
+        out->step_ms_p99 = pct(0.99);
+    }
+    return SPH_OK;
+}
+
+int sph_autotune(sph_ctx* c, int32_t n_steps, sph_tuning* best)
+{
+    if (!c || n_steps < 1) return SPH_E_INVALID;
+    // snapshot the current buffer set and run scalars so timing leaves the state untouched
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    const int cur0 = c->cur;
+    const bool b0 = c->binned, s0 = c->sorted, d0 = c->dens_valid;
+    const size_t half = c->arena_bytes;  // whole arena (simple and correct)
+    void* backup = nullptr;
+    HIPTRY(c, hipMalloc(&backup, half));
+    HIPTRY(c, hipMemcpyAsync(backup, c->arena, half, hipMemcpyDeviceToDevice, c->stream));
+    std::vector<sph_run_config> cfg0 = c->cfg;
+    const sph_tuning t0 = c->tune;
+    double best_t = 1e300;
+    sph_tuning best_tune = t0;
+    float best_cf = -1.0f;
+    hipEvent_t e0, e1;
+    HIPTRY(c, hipEventCreate(&e0));
+    HIPTRY(c, hipEventCreate(&e1));
+    const int dcs[3] = {128, 256, 512}, fcs[3] = {64, 128, 256}, ks[3] = {8, 16, 64};
+    const float cfs[2] = {1.0f, 1.15f};
+    int rc = SPH_OK;
+    for (float cf : cfs) {
+        for (int r = 0; r < c->nruns && !rc; ++r) c->cfg[r].cell_factor = std::max(cf, cfg0[r].cell_factor);
+        for (int dc : dcs)
+            for (int fc : fcs)
+                for (int k : ks) {
+                    if (rc) break;
+                    HIPTRY(c, hipMemcpyAsync(c->arena, backup, half, hipMemcpyDeviceToDevice, c->stream));
+                    c->cur = cur0;
+                    rc = regrid(c);
+                    if (rc) break;
+                    c->tune = {dc, fc, t0.waves_per_cu, k};
+                    rc = do_step(c);  // warm-up
+                    if (rc) break;
+                    HIPTRY(c, hipEventRecord(e0, c->stream));
+                    for (int s = 0; s < n_steps && !rc; ++s) rc = do_step(c);
+                    HIPTRY(c, hipEventRecord(e1, c->stream));
+                    HIPTRY(c, hipEventSynchronize(e1));
+                    float ms = 0;
+                    HIPTRY(c, hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best_t) {
+                        best_t = ms;
+                        best_tune = c->tune;
+                        best_cf = cf;
+                    }
+                }
+    }
+    // restore the state, apply the winner
+    HIPTRY(c, hipMemcpyAsync(c->arena, backup, half, hipMemcpyDeviceToDevice, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    hipFree(backup);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    for (int r = 0; r < c->nruns; ++r) {
+        c->cfg[r] = cfg0[r];
+        if (best_cf > 0) c->cfg[r].cell_factor = std::max(best_cf, cfg0[r].cell_factor);
+    }
+    c->cur = cur0;
+    int rc2 = regrid(c);
+    c->binned = false;  // counts were cleared by regrid
+    c->sorted = false;
+    c->dens_valid = false;
+    (void)b0;
+    (void)s0;
+    (void)d0;
+    c->tune = best_tune;
+    if (best) *best = best_tune;
+    return rc ? rc : rc2;
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+// sph_device.hpp -- device-side data layout and SPH arithmetic of the gfx950 particle step.
+//
+// Layout in HBM (one context = one or more runs, concatenated):
+//   * particle state, structure of arrays, sorted by (supercell key, particle id) after every
+//     resort; two buffer sets (A/B) so the resort gathers without an in-place permutation;
+//   * packed interaction records (float4 planes) written once per pass and staged into LDS:
+//       P0 = (x, y, z, m)   pair coordinate (FP32 absolute or supercell-relative) and mass;
+//                           non-sources carry a sentinel position so the cull rejects them
+//       P1 = (vx, vy, vz, c)         c = fast magnetosonic bound sqrt((gamma P + B^2)/rho)
+//       P2 = (h, 1/h, gfac, rho)     gfac = kernel-gradient prefactor
+//       P3 = (pr, ph, bdr, psr)      pr = (P + B^2/2)/rho^2, ph = P/rho^2, bdr = 1/rho^2, psr = psi/rho^2
+//       P4 = (Bx, By, Bz, psi)       (MHD only)
+//   * supercell table: per-cell particle start (exclusive scan of the per-cell histogram).
+//
+// The arithmetic mirrors the reference's functors: CubicSplineKernel.hpp:47-137,
+// QuinticSplineKernel.hpp:46-160 (shape functions and the gradWScalar guard), DensitySummation.hpp:50-94,
+// HydroForces.hpp:112-175, EquationOfState.hpp:32-47; Wendland C2 per Dehnen & Aly (2012).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sph {
+
+// Per-run constants, uploaded once per composition change.
+struct RunDev {
+    int64_t p0, np;          // first particle, particle count
+    int32_t c0, ncell;       // first supercell, supercell count
+    int32_t n[3];            // supercells per axis
+    int32_t periodic[3];
+    double lo[3], L[3], s[3], inv_s[3];
+    float lo32[3], L32[3], s32[3], inv_s32[3];
+    float H, H2;             // interaction radius and its square (FP32, as used by the cull)
+    int32_t kernel, mhd, cleaning, support;
+    float gamma, alpha, beta, borve, clean_sigma, cfl, cfl_force, sigma;
+    double dt_fixed;
+    int32_t kb_first, kb_count;  // particle-kernel blocks of this run
+    float f0;                    // kernel shape at q = 0
+    int32_t pad;
+};
+
+// Device arrays of one particle buffer set.
+struct StateBuf {
+    uint64_t* id;
+    uint8_t* mask;
+    uint8_t* species;
+    double* pos64[3];  // mixed precision
+    float* pos32[3];   // plain FP32
+    float* mass;
+    float* h;
+    float* u;
+    float* rho;
+    float* psi;
+    float* vel[3];
+    float* B[3];
+};
+
+struct WorkBuf {
+    float* dvdt[3];
+    float* dudt;
+    float* dbdt[3];
+    float* dpsidt;
+    float* divb;
+    float* vsig;
+    uint32_t* nnb;
+    uint32_t* key;
+    uint32_t* rank;
+    uint32_t* tmp;
+    uint32_t* count;
+    uint32_t* cs;     // ncell_tot + 1 entries
+    uint32_t* bsum;
+    float4* P0;
+    float4* P1;
+    float4* P2;
+    float4* P3;
+    float4* P4;
+    double* dt;       // per run: dt of the current step
+    double* time;     // per run
+    uint32_t* chbits; // per run: max fast speed (float bits, atomicMax)
+    double* partial;  // per kick block: 6 doubles (dt, M, Px, Py, Pz, E)
+    double* cons;     // per run: 8 doubles (mass, px, py, pz, energy, dt_next, time, c_h)
+    int32_t* blk_run; // per particle-kernel block: run id
+};
+
+constexpr float kSentinel = 8.507059e37f;  // FLT_MAX / 4: sentinel^2 overflows to +inf, so r2 < H2 fails
+
+__device__ __forceinline__ bool is_target(uint8_t sp, uint8_t live) { return live && sp == 0; }
+__device__ __forceinline__ bool is_source(uint8_t sp, uint8_t live) { return live && sp <= 1; }
+__device__ __forceinline__ bool is_movable(uint8_t sp, uint8_t live) { return live && (sp == 0 || sp == 2); }
+
+// Kernel shape f(q) with W = sigma/h^3 f(q).
+__device__ __forceinline__ float kshape(int kern, float q)
+{
+    if (kern == 2) {  // Wendland C2
+        float t = 1.0f - 0.5f * q;
+        float t2 = t * t;
+        float w = t2 * t2 * (1.0f + 2.0f * q);
+        return q < 2.0f ? w : 0.0f;
+    }
+    if (kern == 0) {  // cubic M4
+        float inner = 1.0f - 1.5f * q * q + 0.75f * q * q * q;
+        float t = 2.0f - q;
+        float outer = 0.25f * t * t * t;
+        return q < 1.0f ? inner : (q < 2.0f ? outer : 0.0f);
+    }
+    // quintic M5
+    float a = 3.0f - q, b = 2.0f - q, c = 1.0f - q;
+    float a5 = a * a * a * a * a, b5 = b * b * b * b * b, c5 = c * c * c * c * c;
+    if (q < 1.0f) return a5 - 6.0f * b5 + 15.0f * c5;
+    if (q < 2.0f) return a5 - 6.0f * b5;
+    if (q < 3.0f) return a5;
+    return 0.0f;
+}
+
+// Gradient scalar s with grad_i W_ij = s r_ij (gfac includes sigma and the h powers).
+__device__ __forceinline__ float kgrad(int kern, float rr, float invR, float q, float gfac)
+{
+    float s;
+    if (kern == 2) {
+        float t = 1.0f - 0.5f * q;
+        s = gfac * (t * t * t);
+        s = q < 2.0f ? s : 0.0f;
+    } else if (kern == 0) {
+        float gi = 2.25f * q * q - 3.0f * q;
+        float t = 2.0f - q;
+        float go = -0.75f * t * t;
+        float g = q < 1.0f ? gi : (q < 2.0f ? go : 0.0f);
+        s = gfac * g * invR;
+    } else {
+        float a = 3.0f - q, b = 2.0f - q, c = 1.0f - q;
+        float a4 = a * a * a * a, b4 = b * b * b * b, c4 = c * c * c * c;
+        float g = q < 1.0f ? (-5.0f * a4 + 30.0f * b4 - 75.0f * c4)
+                           : (q < 2.0f ? (-5.0f * a4 + 30.0f * b4) : (q < 3.0f ? -5.0f * a4 : 0.0f));
+        s = gfac * g * invR;
+    }
+    return rr > 0.0f ? s : 0.0f;
+}
+
+__device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Squared distance from the own pair coordinate to a staged (pre-shifted) neighbour, rounded
+// exactly as ((dx*dx + dy*dy) + dz*dz) with no fused multiply-add: the cull decision and the
+// neighbour counts are part of the bit-exact contract.
+__device__ __forceinline__ float geom_r2(float xi, float yi, float zi, float4 p, float& dx, float& dy, float& dz)
+{
+#pragma clang fp contract(off)
+    dx = xi - p.x;
+    dy = yi - p.y;
+    dz = zi - p.z;
+    float r2 = dx * dx;
+    r2 = r2 + dy * dy;
+    r2 = r2 + dz * dz;
+    return r2;
+}
+
+}  // namespace sph

@@ -1,0 +1,33 @@
+// sph_launch.hpp -- host-side launch interface of the gfx950 kernels (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sph_device.hpp"
+
+namespace sph {
+
+struct LaunchCtx {
+    const RunDev* runs;  // device run table
+    int nruns;
+    int ncell_tot;
+    int nkb;             // particle-kernel blocks (run-aligned, 256 particles each)
+    int mixed;
+    int has_mhd;
+    int density_chunk;
+    int force_chunk;
+    int kmax_log;
+    StateBuf S;          // current buffer set
+    WorkBuf W;
+    hipStream_t stream;
+};
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+hipError_t launch_drift_bin(const LaunchCtx& c, double dt_host, int do_drift);
+hipError_t launch_scan(const LaunchCtx& c);
+hipError_t launch_scatter_gather(const LaunchCtx& c, const StateBuf& Snew);
+hipError_t launch_density(const LaunchCtx& c);
+hipError_t launch_forces(const LaunchCtx& c);
+hipError_t launch_kick(const LaunchCtx& c, double dt_host, int do_kick);
+
+}  // namespace sph

@@ -1,0 +1,217 @@
+"""Python host mirror of the reference's stage interface, over the C ABI.
+
+The reference composes its particle step from stage functors (include/spearhed/control/Simulation.x.cpp:196-310):
+PushVelocity -> UpdateVolumes -> calculateNeighbours -> UpdateDensity -> UpdateHydroForces ->
+EulerIntegrate, followed by one syncWait. The classes below keep those names and that order; each
+one is a thin call into libspearhed_sph.so (no arithmetic happens in Python).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+
+from . import capi
+from .capi import Conserved, GridInfo, RunConfig, Tuning, check
+from .particles import ParticleSet
+
+
+class Context:
+    """A device-resident set of runs (1 = single run, >1 = ragged batch)."""
+
+    def __init__(self, configs: Sequence[RunConfig], sizes: Sequence[int], precision: str = "mixed"):
+        self.lib = capi.load()
+        self.nruns = len(configs)
+        self.precision = capi.PREC_MIXED if precision == "mixed" else capi.PREC_FP32
+        self._cfg = (RunConfig * self.nruns)(*configs)
+        self._cap = (C.c_int64 * self.nruns)(*[int(s) for s in sizes])
+        self.sizes = [int(s) for s in sizes]
+        h = C.c_void_p()
+        rc = self.lib.sph_ctx_create(self._cfg, self._cap, self.nruns, self.precision, None, C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.lib.sph_last_error(h).decode() if h else ""
+            self.close()
+            raise capi.SphError(f"sph_ctx_create: {capi.ERRORS.get(rc, rc)} {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sph_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- data ------------------------------------------------------------------------------------
+    def upload(self, run: int, ps: ParticleSet):
+        s = ps.soa()
+        check(self.lib.sph_upload(self.h, run, C.byref(s)), self.h, "sph_upload")
+
+    def download(self, run: int) -> ParticleSet:
+        ps = ParticleSet(self.sizes[run])
+        s = ps.soa()
+        check(self.lib.sph_download(self.h, run, C.byref(s)), self.h, "sph_download")
+        return ps
+
+    def grid(self, run: int) -> GridInfo:
+        g = GridInfo()
+        check(self.lib.sph_get_grid(self.h, run, C.byref(g)), self.h, "sph_get_grid")
+        return g
+
+    def sort_info(self, run: int):
+        n = self.sizes[run]
+        g = self.grid(run)
+        key = np.zeros(n, dtype=np.uint32)
+        cs = np.zeros(g.n_cells + 1, dtype=np.uint32)
+        check(self.lib.sph_get_sort(self.h, run, key.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                    cs.ctypes.data_as(C.POINTER(C.c_uint32))), self.h, "sph_get_sort")
+        return key, cs
+
+    # -- stages ----------------------------------------------------------------------------------
+    def drift(self, dt: float = 0.0):
+        check(self.lib.sph_stage_drift(self.h, dt), self.h, "sph_stage_drift")
+
+    def resort(self):
+        check(self.lib.sph_stage_resort(self.h), self.h, "sph_stage_resort")
+
+    def neighbour_counts(self):
+        check(self.lib.sph_stage_neighbour_counts(self.h), self.h, "sph_stage_neighbour_counts")
+
+    def density(self):
+        check(self.lib.sph_stage_density(self.h), self.h, "sph_stage_density")
+
+    def forces(self):
+        check(self.lib.sph_stage_forces(self.h), self.h, "sph_stage_forces")
+
+    def integrate(self, dt: float = 0.0):
+        check(self.lib.sph_stage_integrate(self.h, dt), self.h, "sph_stage_integrate")
+
+    def prime(self):
+        check(self.lib.sph_prime(self.h), self.h, "sph_prime")
+
+    def step(self, n: int = 1):
+        check(self.lib.sph_step(self.h, n), self.h, "sph_step")
+
+    def graph_capture(self):
+        check(self.lib.sph_graph_capture(self.h), self.h, "sph_graph_capture")
+
+    def graph_launch(self, n: int):
+        check(self.lib.sph_graph_launch(self.h, n), self.h, "sph_graph_launch")
+
+    def sync(self):
+        check(self.lib.sph_sync(self.h), self.h, "sph_sync")
+
+    def min_dt(self) -> np.ndarray:
+        out = (C.c_double * self.nruns)()
+        check(self.lib.sph_reduce_min_dt(self.h, out), self.h, "sph_reduce_min_dt")
+        return np.array(out[:])
+
+    def conserved(self) -> list:
+        out = (Conserved * self.nruns)()
+        check(self.lib.sph_reduce_conserved(self.h, out), self.h, "sph_reduce_conserved")
+        return [o.as_dict() for o in out]
+
+    def profile_begin(self, max_steps: int):
+        check(self.lib.sph_profile_begin(self.h, max_steps), self.h, "sph_profile_begin")
+
+    def profile_end(self):
+        p = capi.Profile()
+        steps = (C.c_double * 100000)()
+        check(self.lib.sph_profile_end(self.h, C.byref(p), steps), self.h, "sph_profile_end")
+        d = p.as_dict()
+        d["step_ms"] = list(steps[: p.steps])
+        return d
+
+    def neighbour_total(self, run: int) -> int:
+        """Sum of per-particle neighbour counts (accepted pairs) of a run, from the last density pass."""
+        from .capi import HostSoA
+        nnb = np.zeros(self.sizes[run], dtype=np.uint32)
+        s = HostSoA()
+        s.n = self.sizes[run]
+        s.n_neighbours = nnb.ctypes.data_as(C.POINTER(C.c_uint32))
+        check(self.lib.sph_download(self.h, run, C.byref(s)), self.h, "sph_download")
+        return int(nnb.sum(dtype=np.uint64))
+
+    @property
+    def tuning(self) -> dict:
+        t = Tuning()
+        check(self.lib.sph_get_tuning(self.h, C.byref(t)), self.h, "sph_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = Tuning(**{**self.tuning, **kw})
+        check(self.lib.sph_set_tuning(self.h, C.byref(t)), self.h, "sph_set_tuning")
+
+    def autotune(self, n_steps: int = 2) -> dict:
+        t = Tuning()
+        check(self.lib.sph_autotune(self.h, n_steps, C.byref(t)), self.h, "sph_autotune")
+        return t.as_dict()
+
+
+# -- reference-named stage functors ---------------------------------------------------------------
+class PushVelocity:
+    """r += v dt for Movable species (include/spearhed/particles/pusher/PushVelocity.hpp:35-42)."""
+
+    def __call__(self, ctx: Context, dt: float = 0.0):
+        ctx.drift(dt)
+
+
+class UpdateVolumes:
+    """Supercell re-sort; replaces the region AABB refit
+    (packages/pmacc/include/spmacc/particles/regions/RegionBoundsUpdate.hpp:175-214)."""
+
+    def __call__(self, ctx: Context):
+        ctx.resort()
+
+
+class UpdateDensity:
+    """Self term + pair sum (include/spearhed/particles/density/DensitySummation.hpp:104-138)."""
+
+    def __call__(self, ctx: Context):
+        ctx.density()
+
+
+class UpdateHydroForces:
+    """Zero derivatives + pair forces (include/spearhed/sph/HydroForces.hpp:183-214), with viscosity,
+    SPMHD and div-B cleaning when the run enables them."""
+
+    def __call__(self, ctx: Context):
+        ctx.forces()
+
+
+class EulerIntegrate:
+    """v += dvdt dt, u += dudt dt (+ B, psi) and the FP64 CFL / conservation reductions
+    (include/spearhed/particles/pusher/EulerIntegrate.hpp:43-52)."""
+
+    def __call__(self, ctx: Context, dt: float = 0.0):
+        ctx.integrate(dt)
+
+
+class Simulation:
+    """Simulation::runOneStep (include/spearhed/control/Simulation.x.cpp:196-310) over a Context."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.push = PushVelocity()
+        self.update_volumes = UpdateVolumes()
+        self.update_density = UpdateDensity()
+        self.update_hydro = UpdateHydroForces()
+        self.euler = EulerIntegrate()
+
+    def run_one_step(self, dt: float = 0.0):
+        self.push(self.ctx, dt)
+        self.update_volumes(self.ctx)
+        self.update_density(self.ctx)
+        self.update_hydro(self.ctx)
+        self.euler(self.ctx, dt)
+        self.ctx.sync()

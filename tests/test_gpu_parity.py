@@ -1,0 +1,218 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on identical seeded inputs.
+
+Contracts (stated tolerances):
+  * supercell keys, cell starts, the (cell, id) sort permutation and per-particle neighbour counts:
+    bit-exact;
+  * density: relative error <= 1e-5;
+  * dv/dt, du/dt, dB/dt, dpsi/dt, div B: |gpu - oracle| <= 1e-5 * sum_j |term_ij| (the bound of an
+    FP32 summation with cancellation);
+  * FP64 global sums (mass, momentum, energy): <= 1e-6 relative to the sum of magnitudes.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle_ref
+from parity import RTOL, RTOL_GLOBAL, compare_forces, gpu_stages, oracle_stages
+from spearhed_amd import capi, ic
+from spearhed_amd.particles import ParticleSet
+from spearhed_amd.stepper import Context
+
+pytestmark = pytest.mark.gpu
+
+PRECS = ["plain", "mixed"]
+GPREC = {"plain": "fp32", "mixed": "mixed"}
+
+
+def _turb(n=16, seed=11):
+    return ic.mhd_turbulence(n=n, seed=seed)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_sort_bit_exact(prec):
+    cfg, ps = _turb(16, 5)
+    with Context([cfg], [ps.n], precision=GPREC[prec]) as ctx:
+        ctx.upload(0, ps)
+        ctx.resort()
+        key, cs = ctx.sort_info(0)
+        g = ctx.download(0)
+    okey = oracle_ref.bin_keys(cfg, prec, ps.pos)
+    perm, ocs = oracle_ref.sort_perm(okey, ps.id, oracle_ref.grid(cfg).ncell)
+    assert np.array_equal(cs, ocs), "cell starts differ"
+    assert np.array_equal(g.id, ps.id[perm]), "sort permutation differs"
+    assert np.array_equal(key, okey[perm]), "supercell keys differ"
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("problem", ["turb", "kh", "ot_thin"])
+def test_forces_parity_mhd(prec, problem):
+    if problem == "turb":
+        cfg, ps = _turb(16, 3)
+    elif problem == "kh":
+        cfg, ps = ic.kelvin_helmholtz(6000, 9)
+    else:
+        cfg, ps = ic.orszag_tang(n=48, nz=4, seed=2)  # one supercell across z: images at +-L
+    g = gpu_stages(cfg, ps, GPREC[prec])
+    o = oracle_stages(cfg, ps, prec)
+    compare_forces(g, o, mhd=True)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_forces_parity_sod_walls(prec):
+    cfg, ps = ic.sod_tube(nx_left=32, cross=8, seed=4)
+    g = gpu_stages(cfg, ps, GPREC[prec])
+    o = oracle_stages(cfg, ps, prec)
+    compare_forces(g, o, mhd=False)
+
+
+@pytest.mark.parametrize("kernel", [capi.KERNEL_CUBIC, capi.KERNEL_QUINTIC, capi.KERNEL_WENDLAND])
+def test_forces_parity_kernels(kernel):
+    cfg, ps = ic.mhd_turbulence(n=12, seed=21, kernel=kernel)
+    g = gpu_stages(cfg, ps, "mixed")
+    o = oracle_stages(cfg, ps, "mixed")
+    compare_forces(g, o, mhd=True)
+
+
+def test_density_colocated_kat():
+    """DensitySummation.cpp:232-282: N co-located particles have rho = N m W(0, h)."""
+    n, h, m = 4, 0.5, 1.0
+    cfg = capi.make_config(box_lo=(-1, -1, -1), box_len=(2, 2, 2), periodic=(0, 0, 0), kernel=capi.KERNEL_CUBIC,
+                           h0=h, alpha_visc=0.0)
+    ps = ParticleSet(n)
+    ps.mass[:] = m
+    ps.h[:] = h
+    ps.rho[:] = 1
+    ps.u[:] = 1
+    g = gpu_stages(cfg, ps, "fp32", ("resort", "density"))
+    expected = n * m * (1.0 / np.pi) / h ** 3
+    np.testing.assert_allclose(g.rho, expected, rtol=1e-5)
+
+
+def test_density_three_in_line_kat():
+    """DensitySummation.cpp:285-344: particles at x = 0, 1.5, 3 (h = 1, m = 1.5, cubic)."""
+    h, m, dx = 1.0, 1.5, 1.5
+    cfg = capi.make_config(box_lo=(-5, -5, -5), box_len=(12, 10, 10), periodic=(0, 0, 0), kernel=capi.KERNEL_CUBIC,
+                           h0=h, alpha_visc=0.0)
+    ps = ParticleSet(3)
+    ps.pos[0] = [0.0, dx, 2 * dx]
+    ps.mass[:] = m
+    ps.h[:] = h
+    ps.rho[:] = 1
+    ps.u[:] = 1
+    g = gpu_stages(cfg, ps, "fp32", ("resort", "density"))
+    w0 = 1.0 / np.pi
+    q = dx / h
+    w1 = (1.0 / np.pi) * 0.25 * (2 - q) ** 3
+    np.testing.assert_allclose(g.rho, [m * (w0 + w1), m * (w0 + 2 * w1), m * (w0 + w1)], rtol=1e-5)
+
+
+def test_hydro_two_particle_kat():
+    """HydroForces.cpp:159-239: two particles at +-0.2 at rest (h 0.5, u = m = 1, gamma 5/3):
+    dvdt_x = +-2 m P/rho^2 dW/dr(0.4, 0.5), dvdt_y = dvdt_z = 0, dudt = 0. The GPU stage order always
+    runs the density pass first, so rho here is the summed density (identical for both particles)."""
+    h, d, gam = 0.5, 0.4, 5.0 / 3.0
+    cfg = capi.make_config(box_lo=(-2, -2, -2), box_len=(4, 4, 4), periodic=(0, 0, 0), kernel=capi.KERNEL_CUBIC,
+                           h0=h, gamma=gam, alpha_visc=0.0)
+    ps = ParticleSet(2)
+    ps.pos[0] = [-d / 2, d / 2]
+    ps.mass[:] = 1
+    ps.h[:] = h
+    ps.rho[:] = 1
+    ps.u[:] = 1
+    g = gpu_stages(cfg, ps, "fp32")
+    q = d / h
+    w = (1.0 / np.pi) / h ** 3 * (1 - 1.5 * q * q + 0.75 * q ** 3)
+    rho = 1.0 * ((1.0 / np.pi) / h ** 3 + w)
+    np.testing.assert_allclose(g.rho, rho, rtol=1e-5)
+    P = (gam - 1) * rho * 1.0
+    dwdr = (1.0 / np.pi) / h ** 4 * (2.25 * q * q - 3 * q)
+    expected_left = 1.0 * 2 * P / rho ** 2 * dwdr
+    np.testing.assert_allclose(g.dvdt[0], [expected_left, -expected_left], rtol=1e-4)
+    np.testing.assert_allclose(g.dvdt[1:], 0.0, atol=1e-6)
+    np.testing.assert_allclose(g.dudt, 0.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_step_parity(prec):
+    cfg, ps = _turb(16, 17)
+    nsteps = 3
+    with Context([cfg], [ps.n], precision=GPREC[prec]) as ctx:
+        ctx.upload(0, ps)
+        ctx.prime()
+        c0 = ctx.conserved()[0]
+        ctx.step(nsteps)
+        c1 = ctx.conserved()[0]
+        g = ctx.download(0).sorted_by_id()
+    o = oracle_ref.OracleRun(cfg, ps, prec)
+    oc0 = o.prime()
+    assert abs(c0["dt_next"] - oc0["dt_next"]) <= 1e-5 * oc0["dt_next"]
+    oc1 = o.step(nsteps)
+    order = o.by_id()
+    for k in ("mass", "energy"):
+        assert abs(c1[k] - oc1[k]) <= RTOL_GLOBAL * abs(oc1[k]), (k, c1[k], oc1[k])
+    pscale = float(np.sum(o.mass.astype(np.float64) * np.sqrt(np.sum(o.vel.astype(np.float64) ** 2, axis=0))))
+    for a in range(3):
+        assert abs(c1["momentum"][a] - oc1["momentum"][a]) <= RTOL_GLOBAL * pscale
+    # positions after 3 steps: the trajectories agree to FP32 rounding growth
+    dx = np.abs(g.pos - o.pos[:, order])
+    dx = np.minimum(dx, 1.0 - dx)
+    assert float(np.max(dx)) < 1e-5
+    assert float(np.max(np.abs(g.rho - o.rho[order]) / o.rho[order])) < 1e-4
+
+
+def test_graph_replay_matches_eager():
+    cfg, ps = _turb(12, 23)
+    outs = []
+    for use_graph in (False, True):
+        with Context([cfg], [ps.n], precision="mixed") as ctx:
+            ctx.upload(0, ps)
+            ctx.prime()
+            if use_graph:
+                ctx.graph_capture()
+                ctx.graph_launch(2)
+                ctx.sync()
+                ctx.graph_launch(3)
+            else:
+                ctx.step(5)
+            ctx.sync()
+            outs.append((ctx.download(0).sorted_by_id(), ctx.conserved()[0]))
+    (a, ca), (b, cb) = outs
+    for f in ("pos", "vel", "u", "rho", "B", "psi", "dvdt"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    assert ca == cb
+
+
+def test_batched_runs_equal_alone():
+    """Each run of a ragged batch gives bit-identical results to the same run alone."""
+    runs = [ic.kelvin_helmholtz(nt, 100 + r) for r, nt in enumerate([4200, 9000, 5100])]
+    alone = []
+    for cfg, ps in runs:
+        with Context([cfg], [ps.n], precision="mixed") as ctx:
+            ctx.upload(0, ps)
+            ctx.prime()
+            ctx.step(2)
+            ctx.sync()
+            alone.append(ctx.download(0).sorted_by_id())
+    with Context([c for c, _ in runs], [p.n for _, p in runs], precision="mixed") as ctx:
+        for r, (_, ps) in enumerate(runs):
+            ctx.upload(r, ps)
+        ctx.prime()
+        ctx.step(2)
+        ctx.sync()
+        mixed = [ctx.download(r).sorted_by_id() for r in range(len(runs))]
+    for a, b in zip(alone, mixed):
+        for f in ("pos", "vel", "u", "rho", "B", "psi", "dvdt", "dbdt"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+def test_fp64_oracle_error_mixed():
+    """Mixed-precision GPU results against the FP64 oracle, within the stated tolerance."""
+    cfg, ps = _turb(16, 31)
+    g = gpu_stages(cfg, ps, "mixed")
+    o = oracle_stages(cfg, ps, "f64")
+    order = o.by_id()
+    rel_rho = np.abs(g.rho - o.rho[order]) / o.rho[order]
+    assert float(np.max(rel_rho)) < 1e-5
+    e = np.abs(g.dvdt - o.dvdt[:, order]) / o.sc_a[order]
+    assert float(np.max(e)) < 1e-5
