@@ -129,10 +129,11 @@ typedef struct sph_grid_info {
 
 /* Launch tunables chosen by the autotuner (sph_autotune). */
 typedef struct sph_tuning {
-    int32_t density_chunk;  /* neighbours staged in LDS per chunk by the density pass: 128|256|512 */
-    int32_t force_chunk;    /* neighbours staged in LDS per chunk by the force pass: 64|128|256 */
+    int32_t density_chunk;  /* neighbour candidates staged in LDS per chunk by the density pass: 128|256|512 */
+    int32_t force_chunk;    /* neighbour records staged in LDS per chunk by the force pass: 64|128|256 */
     int32_t waves_per_cu;   /* persistent-grid waves per CU: 4..32 */
-    int32_t max_lanes_per_target; /* cap on K, the lanes that split one target's candidates: 1..64 */
+    int32_t max_lanes_per_target; /* cap on K, the lanes that share one target's candidates: 1..64 */
+    int32_t pair_balance;   /* 1: compact each target's accepted pairs and share them among its K lanes */
 } sph_tuning;
 
 typedef struct sph_ctx sph_ctx;
@@ -189,6 +190,10 @@ typedef struct sph_profile {
 int sph_profile_begin(sph_ctx* ctx, int32_t max_steps);
 /* Synchronises; step_ms (may be NULL) receives the per-step durations. */
 int sph_profile_end(sph_ctx* ctx, sph_profile* out, double* step_ms);
+
+/* A plain device-to-device copy of `bytes` (reps times), timed with HIP events: the known-size
+ * reference that HBM counter readings (FETCH_SIZE / WRITE_SIZE) are calibrated against. */
+int sph_probe_copy(int64_t bytes, int32_t reps, double* ms_per_copy);
 
 /* ---- autotuner ------------------------------------------------------------------------------ */
 /* Time n_steps steps for each candidate tuning on the context's current data (state restored

@@ -135,6 +135,7 @@ class Tuning(C.Structure):
         ("force_chunk", C.c_int32),
         ("waves_per_cu", C.c_int32),
         ("max_lanes_per_target", C.c_int32),
+        ("pair_balance", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -190,6 +191,7 @@ PROTOTYPES = {
     "sph_profile_begin": (C.c_int, [_CTX, C.c_int32]),
     "sph_profile_end": (C.c_int, [_CTX, C.POINTER(Profile), C.POINTER(C.c_double)]),
     "sph_autotune": (C.c_int, [_CTX, C.c_int32, C.POINTER(Tuning)]),
+    "sph_probe_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
 }
 
 _lib = None

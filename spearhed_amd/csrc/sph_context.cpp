@@ -35,7 +35,7 @@ struct sph_ctx {
     WorkBuf W = {};
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    sph_tuning tune = {256, 128, 0, 16};
+    sph_tuning tune = {256, 128, 0, 16, 0};
     void* arena = nullptr;
     size_t arena_bytes = 0;
     bool binned = false;   // keys + histogram valid for the current positions
@@ -179,6 +179,8 @@ void carve(Carver& cv, sph_ctx* c, int64_t N, int ncell_cap, int nkb, int nruns)
     W.key = cv.take<uint32_t>(N);
     W.rank = cv.take<uint32_t>(N);
     W.tmp = cv.take<uint32_t>(N);
+    W.tmpid = cv.take<uint64_t>(N);
+    W.nbmask = cv.take<uint32_t>((int64_t)ncell_cap * kMaskWords * 64);
     W.count = cv.take<uint32_t>(ncell_cap);
     W.cs = cv.take<uint32_t>((int64_t)ncell_cap + 1);
     W.bsum = cv.take<uint32_t>((ncell_cap + 2047) / 2048 + 1);
@@ -205,12 +207,10 @@ LaunchCtx lctx(sph_ctx* c)
     L.mixed = c->mixed ? 1 : 0;
     L.has_mhd = c->has_mhd ? 1 : 0;
     L.density_chunk = c->tune.density_chunk;
-    L.force_chunk = c->tune.force_chunk;
-    int k = std::max(1, std::min(64, c->tune.max_lanes_per_target));
-    int lk = 0;
-    while ((2 << lk) <= k) ++lk;
-    L.kmax_log = lk;
+    L.kmax = std::max(1, std::min(64, c->tune.max_lanes_per_target));
     L.S = c->S[c->cur];
+    L.force_chunk = c->tune.force_chunk;
+    L.balance = c->tune.pair_balance ? 1 : 0;
     L.W = c->W;
     L.stream = c->stream;
     return L;
@@ -483,6 +483,7 @@ int sph_set_tuning(sph_ctx* c, const sph_tuning* t)
     if (t->max_lanes_per_target < 1 || t->max_lanes_per_target > 64)
         return fail(c, SPH_E_INVALID, "max_lanes_per_target must be in [1, 64]");
     c->tune = *t;
+    c->dens_valid = false;  // the force pass replays masks laid out for the density pass's tuning
     invalidate_graph(c);
     return SPH_OK;
 }
@@ -505,8 +506,8 @@ int64_t sph_bytes_per_run(const sph_run_config* cfg, int64_t n, int32_t precisio
     const int64_t pos = precision == SPH_PREC_MIXED ? 24 : 12;
     const int64_t mhd = cfg->mhd ? 1 : 0;
     const int64_t state = 8 + 2 + pos + 12 + 16 + mhd * 16;
-    const int64_t work = 12 + 4 + 4 + 4 + 12 + 64 + mhd * (12 + 4 + 4 + 16);
-    return n * (2 * state + work) + (int64_t)R.ncell * 12 + 4096;
+    const int64_t work = 12 + 4 + 4 + 4 + 12 + 8 + 64 + mhd * (12 + 4 + 4 + 16);
+    return n * (2 * state + work) + (int64_t)R.ncell * (12 + 4 * kMaskWords * 64) + 4096;
 }
 
 int sph_upload(sph_ctx* c, int32_t run, const sph_host_soa* s)
@@ -835,20 +836,21 @@ int sph_autotune(sph_ctx* c, int32_t n_steps, sph_tuning* best)
     hipEvent_t e0, e1;
     HIPTRY(c, hipEventCreate(&e0));
     HIPTRY(c, hipEventCreate(&e1));
-    const int dcs[3] = {128, 256, 512}, fcs[3] = {64, 128, 256}, ks[3] = {8, 16, 64};
+    const int dcs[2] = {128, 256}, fcs[2] = {64, 128}, ks[2] = {16, 64}, bals[2] = {0, 1};
     const float cfs[2] = {1.0f, 1.15f};
     int rc = SPH_OK;
     for (float cf : cfs) {
         for (int r = 0; r < c->nruns && !rc; ++r) c->cfg[r].cell_factor = std::max(cf, cfg0[r].cell_factor);
         for (int dc : dcs)
             for (int fc : fcs)
-                for (int k : ks) {
+                for (int k : ks)
+                    for (int bal : bals) {
                     if (rc) break;
                     HIPTRY(c, hipMemcpyAsync(c->arena, backup, half, hipMemcpyDeviceToDevice, c->stream));
                     c->cur = cur0;
                     rc = regrid(c);
                     if (rc) break;
-                    c->tune = {dc, fc, t0.waves_per_cu, k};
+                    c->tune = {dc, fc, t0.waves_per_cu, k, bal};
                     rc = do_step(c);  // warm-up
                     if (rc) break;
                     HIPTRY(c, hipEventRecord(e0, c->stream));

@@ -66,6 +66,9 @@ struct WorkBuf {
     uint32_t* key;
     uint32_t* rank;
     uint32_t* tmp;
+    uint64_t* tmpid;  // particle id staged next to tmp by the scatter (gather ranks by it)
+    uint32_t* nbmask; // accepted-candidate bits per supercell-wave lane: word w of lane l of cell c
+                      // at nbmask[(c * kMaskWords + w) * 64 + l]
     uint32_t* count;
     uint32_t* cs;     // ncell_tot + 1 entries
     uint32_t* bsum;
@@ -83,6 +86,9 @@ struct WorkBuf {
 };
 
 constexpr float kSentinel = 8.507059e37f;  // FLT_MAX / 4: sentinel^2 overflows to +inf, so r2 < H2 fails
+// Mask words kept per lane: the density pass stores which of a lane's first 32*kMaskWords
+// candidates it accepted, and the force pass replays them instead of culling again.
+constexpr int kMaskWords = 8;
 
 __device__ __forceinline__ bool is_target(uint8_t sp, uint8_t live) { return live && sp == 0; }
 __device__ __forceinline__ bool is_source(uint8_t sp, uint8_t live) { return live && sp <= 1; }

@@ -9,14 +9,22 @@
 //   forces        k_forces<CH, MHD>  [UpdateHydroForces, HydroForces.hpp:183-214, + AV/MHD/cleaning]
 //   integrate     k_kick, k_finalize [EulerIntegrate, EulerIntegrate.hpp:43-52, + CFL and FP64 sums]
 //
-// Pair kernels: one wave64 per target supercell. The 27 neighbour supercells form a virtual list
-// that is staged chunk by chunk into LDS (interaction records, pre-shifted by the periodic image or
-// supercell-origin offset, ParticleParticleInteraction.hpp:236-264). The cell's targets are spread
-// over the wave as (target t, sub-lane k) with K = 2^lk sub-lanes per target, so a supercell of
-// ~14 particles still fills the wave; sub-lane k culls candidates k, k+K, ... into a 64-bit mask
-// and then runs the pair physics only over set bits. The K partial sums are combined with a fixed
-// xor-shuffle tree, so every result depends only on the run's own data (bit-identical whether the
-// run is alone or batched with others).
+// Pair passes: one wave64 per target supercell. The 27 neighbour supercells form a virtual list
+// that is staged chunk by chunk into LDS (records pre-shifted by the periodic image or the
+// supercell-origin offset, ParticleParticleInteraction.hpp:236-264). The cell's nt targets are
+// spread over the wave as (target t, sub-lane k) with K = floor(64/nt) sub-lanes per target, so a
+// supercell of ~14 particles still fills the wave; sub-lane k owns candidates v = c*K + k.
+// The density pass culls each lane's candidates 32 at a time into a bit mask (strict r2 < H^2,
+// ParticleParticleInteraction.hpp:291) and stores the masks; the force pass replays them instead
+// of culling again. Each lane then runs the pair physics over its set bits; with pair_balance the
+// accepted candidates of a target's K lanes are first compacted into LDS and shared round-robin
+// among those K lanes (ceil(accepted/K) iterations instead of the slowest lane's count, at the
+// price of a prefix scan and two barriers per 32 candidates). The K partial sums are combined in lane order:
+// every summation order depends only on the run's own data, so a run gives bit-identical results
+// alone or batched with others.
+
+#include <type_traits>
+
 #include "sph_device.hpp"
 #include "sph_launch.hpp"
 
@@ -47,6 +55,17 @@ __device__ __forceinline__ int wave_incl_scan(int v)
     return v;
 }
 
+__device__ __forceinline__ int wave_incl_max(int v)
+{
+    int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(v, off, 64);
+        if (lane >= off) v = v > t ? v : t;
+    }
+    return v;
+}
+
 // XCD-aware mapping: the hardware deals consecutive workgroups round-robin to the 8 XCDs; give
 // each XCD one contiguous range of supercells so that neighbouring cells share an L2.
 __device__ __forceinline__ int xcd_cell(int bid, int nblk_total, int ncell)
@@ -57,7 +76,8 @@ __device__ __forceinline__ int xcd_cell(int bid, int nblk_total, int ncell)
 }
 
 // ------------------------------------------------------------------------------------------------
-// drift + supercell key + histogram rank
+// drift + supercell key + histogram rank (wave-aggregated atomics: particles arrive sorted by
+// cell, so runs of equal keys in a wave take one atomicAdd per run)
 // ------------------------------------------------------------------------------------------------
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_drift_bin(const RunDev* __restrict__ runs, StateBuf S, WorkBuf W,
@@ -67,46 +87,62 @@ __global__ __launch_bounds__(256) void k_drift_bin(const RunDev* __restrict__ ru
     const int r = W.blk_run[b];
     const RunDev& R = runs[r];
     const int64_t local = (int64_t)(b - R.kb_first) * 256 + threadIdx.x;
-    if (local >= R.np) return;
-    const int64_t i = R.p0 + local;
-    const double dt = dt_host > 0.0 ? dt_host : W.dt[r];
-    const uint8_t live = S.mask[i];
-    const bool mov = do_drift && is_movable(S.species[i], live);
-    int c[3];
-    for (int a = 0; a < 3; ++a) {
+    const bool active = local < R.np;
+    const int64_t i = R.p0 + (active ? local : 0);
+    uint32_t key = 0xffffffffu;
+    if (active) {
+        const double dt = dt_host > 0.0 ? dt_host : W.dt[r];
+        const uint8_t live = S.mask[i];
+        const bool mov = do_drift && is_movable(S.species[i], live);
+        int c[3];
+        for (int a = 0; a < 3; ++a) {
 #pragma clang fp contract(off)
-        int ci;
-        if (MIXED) {
-            double x = S.pos64[a][i];
-            if (mov) {
-                x = x + (double)S.vel[a][i] * dt;
-                if (R.periodic[a]) {
-                    double hi = R.lo[a] + R.L[a];
-                    if (x >= hi) x = x - R.L[a];
-                    else if (x < R.lo[a]) x = x + R.L[a];
+            int ci;
+            if (MIXED) {
+                double x = S.pos64[a][i];
+                if (mov) {
+                    x = x + (double)S.vel[a][i] * dt;
+                    if (R.periodic[a]) {
+                        double hi = R.lo[a] + R.L[a];
+                        if (x >= hi) x = x - R.L[a];
+                        else if (x < R.lo[a]) x = x + R.L[a];
+                    }
+                    S.pos64[a][i] = x;
                 }
-                S.pos64[a][i] = x;
-            }
-            ci = (int)floor((x - R.lo[a]) * R.inv_s[a]);
-        } else {
-            float x = S.pos32[a][i];
-            if (mov) {
-                x = x + S.vel[a][i] * (float)dt;
-                if (R.periodic[a]) {
-                    float hi = R.lo32[a] + R.L32[a];
-                    if (x >= hi) x = x - R.L32[a];
-                    else if (x < R.lo32[a]) x = x + R.L32[a];
+                ci = (int)floor((x - R.lo[a]) * R.inv_s[a]);
+            } else {
+                float x = S.pos32[a][i];
+                if (mov) {
+                    x = x + S.vel[a][i] * (float)dt;
+                    if (R.periodic[a]) {
+                        float hi = R.lo32[a] + R.L32[a];
+                        if (x >= hi) x = x - R.L32[a];
+                        else if (x < R.lo32[a]) x = x + R.L32[a];
+                    }
+                    S.pos32[a][i] = x;
                 }
-                S.pos32[a][i] = x;
+                ci = (int)floorf((x - R.lo32[a]) * R.inv_s32[a]);
             }
-            ci = (int)floorf((x - R.lo32[a]) * R.inv_s32[a]);
+            ci = ci < 0 ? 0 : (ci > R.n[a] - 1 ? R.n[a] - 1 : ci);
+            c[a] = ci;
         }
-        ci = ci < 0 ? 0 : (ci > R.n[a] - 1 ? R.n[a] - 1 : ci);
-        c[a] = ci;
+        key = (uint32_t)(R.c0 + c[0] + R.n[0] * (c[1] + R.n[1] * c[2]));
+        W.key[i] = key;
     }
-    const uint32_t key = (uint32_t)(R.c0 + c[0] + R.n[0] * (c[1] + R.n[1] * c[2]));
-    W.key[i] = key;
-    W.rank[i] = atomicAdd(&W.count[key], 1u);
+    // segments of equal consecutive keys within the wave
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || prev != key || !active;
+    const int hl = wave_incl_max(head ? lane : 0);
+    const uint64_t heads = __ballot(head);
+    uint32_t base = 0;
+    if (head && active) {
+        const uint64_t after = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+        const int next = after ? __builtin_ctzll(after) : 64;
+        base = atomicAdd(&W.count[key], (uint32_t)(next - lane));
+    }
+    base = __shfl(base, hl, 64);
+    if (active) W.rank[i] = base + (uint32_t)(lane - hl);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -181,14 +217,16 @@ __global__ __launch_bounds__(256) void k_scan_add(uint32_t* __restrict__ cs, con
 // ------------------------------------------------------------------------------------------------
 // scatter (histogram slot) and gather (final slot = cell start + rank of id within the cell)
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_scatter(const RunDev* __restrict__ runs, WorkBuf W)
+__global__ __launch_bounds__(256) void k_scatter(const RunDev* __restrict__ runs, StateBuf S, WorkBuf W)
 {
     const int b = blockIdx.x;
     const RunDev& R = runs[W.blk_run[b]];
     const int64_t local = (int64_t)(b - R.kb_first) * 256 + threadIdx.x;
     if (local >= R.np) return;
     const int64_t i = R.p0 + local;
-    W.tmp[W.cs[W.key[i]] + W.rank[i]] = (uint32_t)i;
+    const uint32_t slot = W.cs[W.key[i]] + W.rank[i];
+    W.tmp[slot] = (uint32_t)i;
+    W.tmpid[slot] = S.id[i];
 }
 
 template <bool MIXED>
@@ -196,16 +234,18 @@ __global__ __launch_bounds__(256) void k_gather(const RunDev* __restrict__ runs,
                                                 WorkBuf W, int has_mhd)
 {
     const int b = blockIdx.x;
-    const RunDev& R = runs[W.blk_run[b]];
+    const int r = W.blk_run[b];
+    const RunDev& R = runs[r];
     const int64_t local = (int64_t)(b - R.kb_first) * 256 + threadIdx.x;
     if (local >= R.np) return;
     const int64_t q = R.p0 + local;
+    if (local == 0) W.chbits[r] = 0u;  // cleaning-speed maximum of the coming density pass
     const uint32_t i = W.tmp[q];
     const uint32_t c = W.key[i];
     const uint32_t cb = W.cs[c], ce = W.cs[c + 1];
-    const uint64_t myid = Sold.id[i];
+    const uint64_t myid = W.tmpid[q];
     uint32_t rank = 0;
-    for (uint32_t m = cb; m < ce; ++m) rank += (Sold.id[W.tmp[m]] < myid) ? 1u : 0u;
+    for (uint32_t m = cb; m < ce; ++m) rank += (W.tmpid[m] < myid) ? 1u : 0u;
     const uint32_t f = cb + rank;
     if (rank == 0) W.count[c] = 0u;  // histogram reset for the next resort
 
@@ -252,6 +292,31 @@ struct NbList {
     float sh[27][3];
 };
 
+// Neighbour supercell `seg` (0..26, x fastest, d = seg%3-1, seg/3%3-1, seg/9-1) of cell c: returns
+// false outside a non-periodic axis; else the global cell and the staged-position shift (the
+// periodic image for FP32 absolute coordinates, d * cell edge for supercell-relative ones).
+template <bool MIXED>
+__device__ __forceinline__ bool nb_cell(const RunDev& R, const int c[3], int seg, int& cc, float sh[3])
+{
+    const int d[3] = {seg % 3 - 1, (seg / 3) % 3 - 1, seg / 9 - 1};
+    int idx[3];
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+        int x = c[a] + d[a];
+        int wrap = 0;
+        if (R.periodic[a]) {
+            wrap = x < 0 ? -1 : (x >= R.n[a] ? 1 : 0);
+            x -= wrap * R.n[a];
+        } else if (x < 0 || x >= R.n[a]) {
+            ok = false;
+        }
+        idx[a] = x;
+        sh[a] = MIXED ? (float)d[a] * R.s32[a] : (float)wrap * R.L32[a];
+    }
+    cc = R.c0 + idx[0] + R.n[0] * (idx[1] + R.n[1] * idx[2]);
+    return ok;
+}
+
 template <bool MIXED>
 __device__ __forceinline__ void build_nblist(const RunDev& R, int lcell, const uint32_t* __restrict__ cs, NbList& L)
 {
@@ -260,23 +325,8 @@ __device__ __forceinline__ void build_nblist(const RunDev& R, int lcell, const u
     int cnt = 0, st = 0;
     float sh[3] = {0.f, 0.f, 0.f};
     if (lane < 27) {
-        const int d[3] = {lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1};
-        int idx[3];
-        bool ok = true;
-        for (int a = 0; a < 3; ++a) {
-            int x = c[a] + d[a];
-            int wrap = 0;
-            if (R.periodic[a]) {
-                wrap = x < 0 ? -1 : (x >= R.n[a] ? 1 : 0);
-                x -= wrap * R.n[a];
-            } else if (x < 0 || x >= R.n[a]) {
-                ok = false;
-            }
-            idx[a] = x;
-            sh[a] = MIXED ? (float)d[a] * R.s32[a] : (float)wrap * R.L32[a];
-        }
-        if (ok) {
-            int cc = R.c0 + idx[0] + R.n[0] * (idx[1] + R.n[1] * idx[2]);
+        int cc;
+        if (nb_cell<MIXED>(R, c, lane, cc, sh)) {
             st = (int)cs[cc];
             cnt = (int)cs[cc + 1] - st;
         }
@@ -305,26 +355,129 @@ __device__ __forceinline__ int nb_segment(const NbList& L, int v)
     return k;
 }
 
-__device__ __forceinline__ int choose_lk(int nt, int kmax_log)
+// Cull lim (<= 32) consecutive lane candidates starting at LDS slot e0 (stride K): bit ii set iff
+// r2 < H2, read off the sign of r2 - H2 (exact: the rounded difference has the sign of the exact
+// one). Returns LSB-first bits.
+__device__ __forceinline__ uint32_t cull32(const float4* __restrict__ sp, int e0, int K, int lim, float xi, float yi,
+                                           float zi, float H2)
 {
-    // K = 64 / next_pow2(nt), capped at 2^kmax_log; K = 1 for nt >= 64
-    int np2 = 1, lk = 6;
-    while (np2 < nt && lk > 0) {
-        np2 <<= 1;
-        --lk;
+    uint32_t m = 0;
+    if (lim == 32) {
+#pragma unroll
+        for (int ii = 0; ii < 32; ++ii) {
+            float dx, dy, dz;
+            const float r2 = geom_r2(xi, yi, zi, sp[e0 + ii * K], dx, dy, dz);
+            m = (m << 1) | (__float_as_uint(r2 - H2) >> 31);
+        }
+        return __builtin_bitreverse32(m);
     }
-    return lk < kmax_log ? lk : kmax_log;
+    for (int ii = 0; ii < lim; ++ii) {
+        float dx, dy, dz;
+        const float r2 = geom_r2(xi, yi, zi, sp[e0 + ii * K], dx, dy, dz);
+        m = (m << 1) | (__float_as_uint(r2 - H2) >> 31);
+    }
+    return __builtin_bitreverse32(m) >> (32 - lim);
+}
+
+// Lane layout of a supercell-wave: nt targets, K = floor(64/nt) (capped) lanes per target.
+struct LaneMap {
+    int K, tpg, tl, k, gfirst, glast;
+    bool inwave;
+};
+
+__device__ __forceinline__ LaneMap lane_map(int nt, int kmax)
+{
+    LaneMap m;
+    const int lane = threadIdx.x;
+    int K = nt >= 64 ? 1 : 64 / nt;
+    m.K = K < kmax ? K : kmax;
+    m.tpg = 64 / m.K;
+    m.tl = (int)(((float)lane + 0.5f) * (1.0f / (float)m.K));  // lane / K, exact for lane < 64
+    m.k = lane - m.tl * m.K;
+    m.inwave = m.tl < m.tpg;
+    m.gfirst = m.inwave ? m.tl * m.K : 63;
+    m.glast = m.inwave ? m.gfirst + m.K - 1 : 63;
+    return m;
+}
+
+// Whether a supercell-wave keeps its masks: one target group and every lane's candidates fit.
+__device__ __forceinline__ bool masks_fit(int nt, int total, const LaneMap& m)
+{
+    return nt <= m.tpg && (total + m.K - 1) / m.K <= 32 * kMaskWords;
+}
+
+// Mask words are indexed by lane candidate c (candidate v = c*K + k of sub-lane k), so the two
+// passes may stage with different chunk sizes. Bits for [c0, c0 + lim) may straddle two words.
+struct MaskWriter {
+    uint32_t* row;
+    uint32_t pend;
+    int pend_w;
+    __device__ __forceinline__ void put(uint32_t bits, int c0, int lim)
+    {
+        const int w = c0 >> 5, off = c0 & 31;
+        if (w >= kMaskWords) return;
+        if (w != pend_w) {
+            if (pend_w >= 0) row[pend_w * 64] = pend;
+            pend = 0;
+            pend_w = w;
+        }
+        pend |= bits << off;
+        if (off + lim > 32) {
+            row[w * 64] = pend;
+            pend = bits >> (32 - off);
+            pend_w = w + 1 < kMaskWords ? w + 1 : -1;
+        }
+    }
+    __device__ __forceinline__ void flush()
+    {
+        if (pend_w >= 0) row[pend_w * 64] = pend;
+        pend_w = -1;
+    }
+};
+
+__device__ __forceinline__ uint32_t mask_get(const uint32_t* row, int c0, int lim, int per_lane)
+{
+    int nvalid = per_lane - c0;
+    if (nvalid <= 0) return 0u;
+    if (nvalid > lim) nvalid = lim;
+    const int w = c0 >> 5, off = c0 & 31;
+    if (w >= kMaskWords) return 0u;
+    uint32_t a = row[w * 64] >> off;
+    if (off && off + nvalid > 32 && w + 1 < kMaskWords) a |= row[(w + 1) * 64] << (32 - off);
+    return nvalid == 32 ? a : (a & ((1u << nvalid) - 1u));
+}
+
+// Compact the accepted LDS slots of each target's K lanes into lst[gfirst*32 ...) (in sub-lane,
+// then candidate order) and return the target's count; the K lanes then share them round-robin.
+template <class LT>
+__device__ __forceinline__ int compact_group(uint32_t msk, int sb, const LaneMap& m, LT* lst)
+{
+    const int na = __popc(msk);
+    const int incl = wave_incl_scan(na);
+    const int bprev = __shfl(incl, m.gfirst > 0 ? m.gfirst - 1 : 0, 64);
+    const int before = m.gfirst > 0 ? bprev : 0;
+    const int gtot = __shfl(incl, m.glast, 64) - before;
+    int wpos = m.gfirst * 32 + (incl - na - before);
+    while (msk) {
+        const int ii = __builtin_ctz(msk);
+        msk &= msk - 1;
+        lst[wpos++] = (LT)(m.k + (sb + ii) * m.K);
+    }
+    return m.inwave ? gtot : 0;
 }
 
 // ------------------------------------------------------------------------------------------------
-// density pass (+ per-particle interaction records for the force pass)
+// density pass: cull of the 27-supercell candidates staged in LDS, density sum, candidate masks,
+// and the per-particle interaction records of the force pass
 // ------------------------------------------------------------------------------------------------
 template <int CH, bool MIXED, bool MHD>
 __global__ __launch_bounds__(64) void k_density(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
-                                                int nblk_total, StateBuf S, WorkBuf W, int kmax_log)
+                                                int nblk_total, StateBuf S, WorkBuf W, int kmax, int balance)
 {
     __shared__ NbList L;
     __shared__ float4 sp[CH];
+    using LT = typename std::conditional<(CH <= 256), uint8_t, uint16_t>::type;
+    __shared__ LT lst[64 * 32];
     const int cell = xcd_cell(blockIdx.x, nblk_total, ncell_tot);
     if (cell < 0) return;
     const int t0 = (int)W.cs[cell];
@@ -336,29 +489,45 @@ __global__ __launch_bounds__(64) void k_density(const RunDev* __restrict__ runs,
     const int total = L.off[27];
     const int self_v0 = L.off[13];
     const int lane = threadIdx.x;
-    const int lk = choose_lk(nt, kmax_log);
-    const int K = 1 << lk;
-    const int tpg = 64 >> lk;
+    const LaneMap lm = lane_map(nt, kmax);
+    const int K = lm.K, k = lm.k;
+    const int per = CH / K;   // lane candidates per chunk
+    const int chk = per * K;  // staged entries per chunk (a multiple of K)
     const int kern = R.kernel;
     const float H2 = R.H2;
     const float gam = R.gamma;
+    const bool keep = masks_fit(nt, total, lm);
+    MaskWriter mw;
+    mw.row = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
+    mw.pend = 0;
+    mw.pend_w = -1;
 
-    for (int tg = 0; tg < nt; tg += tpg) {
-        const int t = tg + (lane >> lk);
-        const int k = lane & (K - 1);
-        const bool valid = t < nt;
+    for (int tg = 0; tg < nt; tg += lm.tpg) {
+        const int t = tg + lm.tl;
+        const bool valid = lm.inwave && t < nt;
         const int i = t0 + (valid ? t : 0);
         const uint8_t live = S.mask[i], spc = S.species[i];
         const bool tgt = valid && is_target(spc, live);
         const float4 own = W.P0[i];
         const float h = S.h[i];
+        // inputs of the epilogue, loaded early so their latency overlaps the cull
+        const float uu = S.u[i];
+        const float vx = S.vel[0][i], vy = S.vel[1][i], vz = S.vel[2][i];
+        const float rho_in = S.rho[i];
+        float bx = 0.f, by = 0.f, bz = 0.f, psi = 0.f;
+        if (MHD && R.mhd) {
+            bx = S.B[0][i];
+            by = S.B[1][i];
+            bz = S.B[2][i];
+            psi = S.psi[i];
+        }
         const float invh = 1.0f / h;
         const int self_v = self_v0 + t;
         float acc = 0.0f;
         uint32_t cnt = 0;
 
-        for (int base = 0; base < total; base += CH) {
-            for (int e = lane; e < CH; e += 64) {
+        for (int base = 0; base < total; base += chk) {
+            for (int e = lane; e < chk; e += 64) {
                 int v = base + e;
                 float4 p = make_float4(kSentinel, kSentinel, kSentinel, 0.0f);
                 if (v < total) {
@@ -371,36 +540,53 @@ __global__ __launch_bounds__(64) void k_density(const RunDev* __restrict__ runs,
                 sp[e] = p;
             }
             __syncthreads();
-            const int per = CH >> lk;
-            for (int sb = 0; sb < per; sb += 64) {
-                const int lim = per - sb < 64 ? per - sb : 64;
-                uint64_t msk = 0;
-                for (int ii = 0; ii < lim; ++ii) {
-                    const int e = k + ((sb + ii) << lk);
-                    float dx, dy, dz;
-                    const float r2 = geom_r2(own.x, own.y, own.z, sp[e], dx, dy, dz);
-                    const bool ok = (r2 < H2) && (base + e != self_v);
-                    msk |= (uint64_t)ok << ii;
+            const int cbase = base / K;
+            for (int sb = 0; sb < per; sb += 32) {
+                const int lim = per - sb < 32 ? per - sb : 32;
+                uint32_t msk = cull32(sp, k + sb * K, K, lim, own.x, own.y, own.z, H2);
+                // the self pair (r2 = 0) passes the cull: drop it here; its density term is the
+                // self term m_i W(0, h_i) added below, as DensityInitSelf does
+                const int sc = self_v - k;
+                if (sc >= 0 && sc % K == 0) {
+                    const int bit = sc / K - cbase - sb;
+                    if (bit >= 0 && bit < lim) msk &= ~(1u << bit);
                 }
                 if (!tgt) msk = 0;
-                cnt += (uint32_t)__popcll(msk);
-                while (msk) {
-                    const int ii = __builtin_ctzll(msk);
-                    msk &= msk - 1;
-                    const int e = k + ((sb + ii) << lk);
-                    const float4 p = sp[e];
-                    float dx, dy, dz;
-                    const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
-                    const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
-                    const float rr = r2 * invR;
-                    acc += p.w * kshape(kern, rr * invh);
+                if (keep) mw.put(msk, cbase + sb, lim);
+                cnt += (uint32_t)__popc(msk);
+                if (K == 1 || !balance) {
+                    while (msk) {
+                        const int ii = __builtin_ctz(msk);
+                        msk &= msk - 1;
+                        const float4 p = sp[k + (sb + ii) * K];
+                        float dx, dy, dz;
+                        const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
+                        const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
+                        acc += p.w * kshape(kern, r2 * invR * invh);
+                    }
+                } else {
+                    const int gtot = compact_group(msk, sb, lm, lst);
+                    __syncthreads();
+                    for (int q = k; q < gtot; q += K) {
+                        const float4 p = sp[lst[lm.gfirst * 32 + q]];
+                        float dx, dy, dz;
+                        const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
+                        const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
+                        acc += p.w * kshape(kern, r2 * invR * invh);
+                    }
+                    __syncthreads();
                 }
             }
             __syncthreads();
         }
-        for (int off = 1; off < K; off <<= 1) {
-            acc += __shfl_xor(acc, off, 64);
-            cnt += __shfl_xor(cnt, off, 64);
+        if (keep) mw.flush();
+        // combine the K partial sums in lane order (deterministic)
+        float acc_t = 0.0f;
+        uint32_t cnt_t = 0;
+        for (int kk = 0; kk < K; ++kk) {
+            const int src = lm.gfirst + kk < 64 ? lm.gfirst + kk : 63;
+            acc_t += __shfl(acc, src, 64);
+            cnt_t += __shfl(cnt, src, 64);
         }
         // epilogue: one lane per particle writes rho and its interaction records
         float cmax = 0.0f;
@@ -411,29 +597,21 @@ __global__ __launch_bounds__(64) void k_density(const RunDev* __restrict__ runs,
             const float gfac = (kern == 2) ? -5.0f * sig / (h * h * h * h * h) : sig / (h * h * h * h);
             float rho;
             if (tgt) {
-                rho = wfac * (acc + own.w * R.f0);
+                rho = wfac * (acc_t + own.w * R.f0);
                 S.rho[i] = rho;
-                W.nnb[i] = cnt;
+                W.nnb[i] = cnt_t;
             } else {
-                rho = S.rho[i];
+                rho = rho_in;
             }
-            const float uu = S.u[i];
             const float P = (gam - 1.0f) * rho * uu;
             const float invrho = 1.0f / rho;
-            float B2 = 0.0f, bx = 0.f, by = 0.f, bz = 0.f, psi = 0.f;
-            if (MHD && R.mhd) {
-                bx = S.B[0][i];
-                by = S.B[1][i];
-                bz = S.B[2][i];
-                psi = S.psi[i];
-                B2 = bx * bx + by * by + bz * bz;
-            }
+            const float B2 = bx * bx + by * by + bz * bz;
             const float bdr = invrho * invrho;
             const float ph = P * bdr;
             const float pr = (P + 0.5f * B2) * bdr;
             const float psr = psi * bdr;
             const float c = sqrtf((gam * P + B2) * invrho);
-            W.P1[i] = make_float4(S.vel[0][i], S.vel[1][i], S.vel[2][i], c);
+            W.P1[i] = make_float4(vx, vy, vz, c);
             W.P2[i] = make_float4(h, invh, gfac, rho);
             W.P3[i] = make_float4(pr, ph, bdr, psr);
             if (MHD) W.P4[i] = make_float4(bx, by, bz, psi);
@@ -442,7 +620,13 @@ __global__ __launch_bounds__(64) void k_density(const RunDev* __restrict__ runs,
         if (k == 0 && valid && !tgt) W.nnb[i] = 0;
         if (MHD && R.mhd && R.cleaning) {
             for (int off = 32; off >= 1; off >>= 1) cmax = fmaxf(cmax, __shfl_xor(cmax, off, 64));
-            if (lane == 0 && cmax > 0.0f) atomicMax(&W.chbits[r], __float_as_uint(cmax));
+            // every wave of the run targets this one word: test before the atomic, so that only a
+            // wave holding a new maximum pays for one (the value only grows, so a stale read costs
+            // at most a redundant atomic)
+            if (lane == 0 && cmax > 0.0f) {
+                const uint32_t bits = __float_as_uint(cmax);
+                if (bits > __atomic_load_n(&W.chbits[r], __ATOMIC_RELAXED)) atomicMax(&W.chbits[r], bits);
+            }
         }
     }
 }
@@ -455,13 +639,75 @@ struct ForceAcc {
     float Ti, Tj, TBx, TBy, TBz, VBx, VBy, VBz, DB, PSx, PSy, PSz;
 };
 
+template <bool MHD>
+__device__ __forceinline__ void force_pair(const float4& o0, const float4& o1, const float4& o2, const float4& o3,
+                                           const float4& o4, const float4& q0, const float4& q1, const float4& q2,
+                                           const float4& q3, const float4& q4, int kern, float alpha, float beta,
+                                           bool mhd, bool clean, ForceAcc& A)
+{
+    float dx, dy, dz;
+    const float r2 = geom_r2(o0.x, o0.y, o0.z, q0, dx, dy, dz);
+    const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
+    const float rr = r2 * invR;
+    const float s_i = kgrad(kern, rr, invR, rr * o2.y, o2.z);
+    const float s_j = kgrad(kern, rr, invR, rr * q2.y, q2.z);
+    const float dvx = o1.x - q1.x, dvy = o1.y - q1.y, dvz = o1.z - q1.z;
+    const float vr = dvx * dx + dvy * dy + dvz * dz;
+    const float m = q0.w;
+    const float ms_i = m * s_i, ms_j = m * s_j;
+    const float fiso = o3.x * ms_i + q3.x * ms_j;
+    A.ax -= fiso * dx;
+    A.ay -= fiso * dy;
+    A.az -= fiso * dz;
+    A.DV += ms_i * vr;
+    const float w = vr * invR;
+    const float vs = o1.w + q1.w - beta * fminf(w, 0.0f);
+    A.vmax = fmaxf(A.vmax, vs);
+    if (alpha > 0.0f && vr < 0.0f) {
+        const float hb = 0.5f * (o2.x + q2.x);
+        const float mu = hb * vr * fast_rcp(r2 + 0.01f * hb * hb);
+        const float cb = 0.5f * (o1.w + q1.w);
+        const float rb = 0.5f * (o2.w + q2.w);
+        const float Pi = (beta * mu - alpha * cb) * mu * fast_rcp(rb);
+        const float gv = Pi * 0.5f * (ms_i + ms_j);
+        A.ax -= gv * dx;
+        A.ay -= gv * dy;
+        A.az -= gv * dz;
+        A.AVU += gv * vr;
+    }
+    if (MHD && mhd) {
+        const float Bir = o4.x * dx + o4.y * dy + o4.z * dz;
+        const float Bjr = q4.x * dx + q4.y * dy + q4.z * dz;
+        const float ti = Bir * o3.z * ms_i;
+        const float tj = Bjr * q3.z * ms_j;
+        A.Ti += ti;
+        A.Tj += tj;
+        A.TBx += tj * q4.x;
+        A.TBy += tj * q4.y;
+        A.TBz += tj * q4.z;
+        const float wv = ms_i * Bir;
+        A.VBx += wv * dvx;
+        A.VBy += wv * dvy;
+        A.VBz += wv * dvz;
+        A.DB += ms_i * (Bir - Bjr);
+        if (clean) {
+            const float fps = o3.w * ms_i + q3.w * ms_j;
+            A.PSx += fps * dx;
+            A.PSy += fps * dy;
+            A.PSz += fps * dz;
+        }
+    }
+}
+
 template <int CH, bool MIXED, bool MHD>
 __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
-                                               int nblk_total, StateBuf S, WorkBuf W, int kmax_log)
+                                               int nblk_total, StateBuf S, WorkBuf W, int kmax, int balance)
 {
     constexpr int NP = MHD ? 5 : 4;
     __shared__ NbList L;
     __shared__ float4 sp[NP][CH];
+    using LT = typename std::conditional<(CH <= 256), uint8_t, uint16_t>::type;
+    __shared__ LT lst[64 * 32];
     const int cell = xcd_cell(blockIdx.x, nblk_total, ncell_tot);
     if (cell < 0) return;
     const int t0 = (int)W.cs[cell];
@@ -471,11 +717,11 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
     const RunDev& R = runs[r];
     build_nblist<MIXED>(R, cell - R.c0, W.cs, L);
     const int total = L.off[27];
-    const int self_v0 = L.off[13];
     const int lane = threadIdx.x;
-    const int lk = choose_lk(nt, kmax_log);
-    const int K = 1 << lk;
-    const int tpg = 64 >> lk;
+    const LaneMap lm = lane_map(nt, kmax);
+    const int K = lm.K, k = lm.k;
+    const int per = CH / K;
+    const int chk = per * K;
     const int kern = R.kernel;
     const float H2 = R.H2;
     const float alpha = R.alpha, beta = R.beta;
@@ -483,21 +729,23 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
     const bool clean = mhd && R.cleaning;
     const float borve = mhd ? R.borve : 0.0f;
     const float ch = __uint_as_float(W.chbits[r]);
+    const bool keep = masks_fit(nt, total, lm);
+    const int per_lane = (total + K - 1) / K;
+    const uint32_t* mrow = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
 
-    for (int tg = 0; tg < nt; tg += tpg) {
-        const int t = tg + (lane >> lk);
-        const int k = lane & (K - 1);
-        const bool valid = t < nt;
+    for (int tg = 0; tg < nt; tg += lm.tpg) {
+        const int t = tg + lm.tl;
+        const bool valid = lm.inwave && t < nt;
         const int i = t0 + (valid ? t : 0);
         const bool tgt = valid && is_target(S.species[i], S.mask[i]);
         const float4 o0 = W.P0[i], o1 = W.P1[i], o2 = W.P2[i], o3 = W.P3[i];
         float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f);
         if (MHD) o4 = W.P4[i];
-        const int self_v = self_v0 + t;
+        const int self_v = L.off[13] + t;
         ForceAcc A = {};
 
-        for (int base = 0; base < total; base += CH) {
-            for (int e = lane; e < CH; e += 64) {
+        for (int base = 0; base < total; base += chk) {
+            for (int e = lane; e < chk; e += 64) {
                 int v = base + e;
                 if (v < total) {
                     int sg = nb_segment(L, v);
@@ -516,128 +764,103 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
                 }
             }
             __syncthreads();
-            const int per = CH >> lk;
-            for (int sb = 0; sb < per; sb += 64) {
-                const int lim = per - sb < 64 ? per - sb : 64;
-                uint64_t msk = 0;
-                for (int ii = 0; ii < lim; ++ii) {
-                    const int e = k + ((sb + ii) << lk);
-                    float dx, dy, dz;
-                    const float r2 = geom_r2(o0.x, o0.y, o0.z, sp[0][e], dx, dy, dz);
-                    const bool ok = (r2 < H2) && (base + e != self_v);
-                    msk |= (uint64_t)ok << ii;
+            const int cbase = base / K;
+            for (int sb = 0; sb < per; sb += 32) {
+                const int lim = per - sb < 32 ? per - sb : 32;
+                uint32_t msk;
+                if (keep) {
+                    msk = mask_get(mrow, cbase + sb, lim, per_lane);
+                } else {
+                    msk = cull32(sp[0], k + sb * K, K, lim, o0.x, o0.y, o0.z, H2);
+                    const int sc = self_v - k;  // the self pair adds exact zeros; drop it anyway
+                    if (sc >= 0 && sc % K == 0) {
+                        const int bit = sc / K - cbase - sb;
+                        if (bit >= 0 && bit < lim) msk &= ~(1u << bit);
+                    }
                 }
                 if (!tgt) msk = 0;
-                while (msk) {
-                    const int ii = __builtin_ctzll(msk);
-                    msk &= msk - 1;
-                    const int e = k + ((sb + ii) << lk);
-                    const float4 q0 = sp[0][e], q1 = sp[1][e], q2 = sp[2][e], q3 = sp[3][e];
-                    float dx, dy, dz;
-                    const float r2 = geom_r2(o0.x, o0.y, o0.z, q0, dx, dy, dz);
-                    const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
-                    const float rr = r2 * invR;
-                    const float s_i = kgrad(kern, rr, invR, rr * o2.y, o2.z);
-                    const float s_j = kgrad(kern, rr, invR, rr * q2.y, q2.z);
-                    const float dvx = o1.x - q1.x, dvy = o1.y - q1.y, dvz = o1.z - q1.z;
-                    const float vr = dvx * dx + dvy * dy + dvz * dz;
-                    const float m = q0.w;
-                    const float ms_i = m * s_i, ms_j = m * s_j;
-                    const float fiso = o3.x * ms_i + q3.x * ms_j;
-                    A.ax -= fiso * dx;
-                    A.ay -= fiso * dy;
-                    A.az -= fiso * dz;
-                    A.DV += ms_i * vr;
-                    const float w = vr * invR;
-                    const float vs = o1.w + q1.w - beta * fminf(w, 0.0f);
-                    A.vmax = fmaxf(A.vmax, vs);
-                    if (alpha > 0.0f && vr < 0.0f) {
-                        const float hb = 0.5f * (o2.x + q2.x);
-                        const float mu = hb * vr * fast_rcp(r2 + 0.01f * hb * hb);
-                        const float cb = 0.5f * (o1.w + q1.w);
-                        const float rb = 0.5f * (o2.w + q2.w);
-                        const float Pi = (beta * mu - alpha * cb) * mu * fast_rcp(rb);
-                        const float gv = Pi * 0.5f * (ms_i + ms_j);
-                        A.ax -= gv * dx;
-                        A.ay -= gv * dy;
-                        A.az -= gv * dz;
-                        A.AVU += gv * vr;
+                if (K == 1 || !balance) {
+                    while (msk) {
+                        const int ii = __builtin_ctz(msk);
+                        msk &= msk - 1;
+                        const int e = k + (sb + ii) * K;
+                        float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (MHD) q4 = sp[4][e];
+                        force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
+                                        beta, mhd, clean, A);
                     }
-                    if (MHD && mhd) {
-                        const float4 q4 = sp[4][e];
-                        const float Bir = o4.x * dx + o4.y * dy + o4.z * dz;
-                        const float Bjr = q4.x * dx + q4.y * dy + q4.z * dz;
-                        const float ti = Bir * o3.z * ms_i;
-                        const float tj = Bjr * q3.z * ms_j;
-                        A.Ti += ti;
-                        A.Tj += tj;
-                        A.TBx += tj * q4.x;
-                        A.TBy += tj * q4.y;
-                        A.TBz += tj * q4.z;
-                        const float wv = ms_i * Bir;
-                        A.VBx += wv * dvx;
-                        A.VBy += wv * dvy;
-                        A.VBz += wv * dvz;
-                        A.DB += ms_i * (Bir - Bjr);
-                        if (clean) {
-                            const float fps = o3.w * ms_i + q3.w * ms_j;
-                            A.PSx += fps * dx;
-                            A.PSy += fps * dy;
-                            A.PSz += fps * dz;
-                        }
+                } else {
+                    const int gtot = compact_group(msk, sb, lm, lst);
+                    __syncthreads();
+                    for (int q = k; q < gtot; q += K) {
+                        const int e = lst[lm.gfirst * 32 + q];
+                        float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (MHD) q4 = sp[4][e];
+                        force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
+                                        beta, mhd, clean, A);
                     }
+                    __syncthreads();
                 }
             }
             __syncthreads();
         }
-        for (int off = 1; off < K; off <<= 1) {
-            A.ax += __shfl_xor(A.ax, off, 64);
-            A.ay += __shfl_xor(A.ay, off, 64);
-            A.az += __shfl_xor(A.az, off, 64);
-            A.DV += __shfl_xor(A.DV, off, 64);
-            A.AVU += __shfl_xor(A.AVU, off, 64);
-            A.vmax = fmaxf(A.vmax, __shfl_xor(A.vmax, off, 64));
-            if (MHD) {
-                A.Ti += __shfl_xor(A.Ti, off, 64);
-                A.Tj += __shfl_xor(A.Tj, off, 64);
-                A.TBx += __shfl_xor(A.TBx, off, 64);
-                A.TBy += __shfl_xor(A.TBy, off, 64);
-                A.TBz += __shfl_xor(A.TBz, off, 64);
-                A.VBx += __shfl_xor(A.VBx, off, 64);
-                A.VBy += __shfl_xor(A.VBy, off, 64);
-                A.VBz += __shfl_xor(A.VBz, off, 64);
-                A.DB += __shfl_xor(A.DB, off, 64);
-                A.PSx += __shfl_xor(A.PSx, off, 64);
-                A.PSy += __shfl_xor(A.PSy, off, 64);
-                A.PSz += __shfl_xor(A.PSz, off, 64);
-            }
+        // combine the K partial sums in lane order (deterministic), field by field in place
+        auto red = [&](float v) {
+            float s = 0.0f;
+            for (int kk = 0; kk < K; ++kk) s += __shfl(v, lm.gfirst + kk < 64 ? lm.gfirst + kk : 63, 64);
+            return s;
+        };
+        ForceAcc& T = A;
+        T.ax = red(A.ax);
+        T.ay = red(A.ay);
+        T.az = red(A.az);
+        T.DV = red(A.DV);
+        T.AVU = red(A.AVU);
+        {
+            float vm = 0.0f;
+            for (int kk = 0; kk < K; ++kk) vm = fmaxf(vm, __shfl(A.vmax, lm.gfirst + kk < 64 ? lm.gfirst + kk : 63, 64));
+            T.vmax = vm;
+        }
+        if (MHD) {
+            T.Ti = red(A.Ti);
+            T.Tj = red(A.Tj);
+            T.TBx = red(A.TBx);
+            T.TBy = red(A.TBy);
+            T.TBz = red(A.TBz);
+            T.VBx = red(A.VBx);
+            T.VBy = red(A.VBy);
+            T.VBz = red(A.VBz);
+            T.DB = red(A.DB);
+            T.PSx = red(A.PSx);
+            T.PSy = red(A.PSy);
+            T.PSz = red(A.PSz);
         }
         if (k == 0 && tgt) {
             const float rho = o2.w;
             const float invrho = 1.0f / rho;
-            float ax = A.ax, ay = A.ay, az = A.az;
+            float ax = T.ax, ay = T.ay, az = T.az;
             if (MHD && mhd) {
-                const float tcoef = A.Ti - borve * (A.Ti + A.Tj);
-                ax = ax + A.TBx + o4.x * tcoef;
-                ay = ay + A.TBy + o4.y * tcoef;
-                az = az + A.TBz + o4.z * tcoef;
+                const float tcoef = T.Ti - borve * (T.Ti + T.Tj);
+                ax = ax + T.TBx + o4.x * tcoef;
+                ay = ay + T.TBy + o4.y * tcoef;
+                az = az + T.TBz + o4.z * tcoef;
             }
             W.dvdt[0][i] = ax;
             W.dvdt[1][i] = ay;
             W.dvdt[2][i] = az;
-            W.dudt[i] = o3.y * A.DV + 0.5f * A.AVU;
-            W.vsig[i] = fmaxf(2.0f * o1.w, A.vmax);
+            W.dudt[i] = o3.y * T.DV + 0.5f * T.AVU;
+            W.vsig[i] = fmaxf(2.0f * o1.w, T.vmax);
             if (MHD && mhd) {
-                const float divB = -A.DB * invrho;
-                const float divv = -A.DV * invrho;
-                float dbx = -(A.VBx - o4.x * A.DV) * invrho;
-                float dby = -(A.VBy - o4.y * A.DV) * invrho;
-                float dbz = -(A.VBz - o4.z * A.DV) * invrho;
+                const float divB = -T.DB * invrho;
+                const float divv = -T.DV * invrho;
+                float dbx = -(T.VBx - o4.x * T.DV) * invrho;
+                float dby = -(T.VBy - o4.y * T.DV) * invrho;
+                float dbz = -(T.VBz - o4.z * T.DV) * invrho;
                 float dpsi = 0.0f;
                 if (clean) {
-                    dbx -= rho * A.PSx;
-                    dby -= rho * A.PSy;
-                    dbz -= rho * A.PSz;
+                    dbx -= rho * T.PSx;
+                    dby -= rho * T.PSy;
+                    dbz -= rho * T.PSz;
                     const float psi = o4.w;
                     dpsi = -ch * ch * divB - psi * R.clean_sigma * ch * o2.y - 0.5f * psi * divv;
                 }
@@ -748,7 +971,7 @@ __global__ __launch_bounds__(256) void k_kick(const RunDev* __restrict__ runs, S
 }
 
 // One block per run: reduce the run's kick-block partials in a fixed tree order, publish the
-// conserved sums, set the next dt, advance the time and clear the cleaning-speed maximum.
+// conserved sums, set the next dt and advance the time.
 __global__ __launch_bounds__(256) void k_finalize(const RunDev* __restrict__ runs, WorkBuf W, double dt_host,
                                                   int advance)
 {
@@ -786,7 +1009,6 @@ __global__ __launch_bounds__(256) void k_finalize(const RunDev* __restrict__ run
         c[6] = t;
         c[7] = (double)__uint_as_float(W.chbits[r]);
         W.dt[r] = dtn;
-        W.chbits[r] = 0u;
     }
 }
 
@@ -829,7 +1051,7 @@ hipError_t launch_scan(const LaunchCtx& c)
 
 hipError_t launch_scatter_gather(const LaunchCtx& c, const StateBuf& Snew)
 {
-    k_scatter<<<c.nkb, 256, 0, c.stream>>>(c.runs, c.W);
+    k_scatter<<<c.nkb, 256, 0, c.stream>>>(c.runs, c.S, c.W);
     if (c.mixed)
         k_gather<true><<<c.nkb, 256, 0, c.stream>>>(c.runs, c.S, Snew, c.W, c.has_mhd);
     else
@@ -842,14 +1064,14 @@ static void density_ch(const LaunchCtx& c, int nblk)
 {
     if (c.mixed) {
         if (c.has_mhd)
-            k_density<CH, true, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax_log);
+            k_density<CH, true, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
         else
-            k_density<CH, true, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax_log);
+            k_density<CH, true, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
     } else {
         if (c.has_mhd)
-            k_density<CH, false, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax_log);
+            k_density<CH, false, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
         else
-            k_density<CH, false, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax_log);
+            k_density<CH, false, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
     }
 }
 
@@ -869,14 +1091,14 @@ static void forces_ch(const LaunchCtx& c, int nblk)
 {
     if (c.mixed) {
         if (c.has_mhd)
-            k_forces<CH, true, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax_log);
+            k_forces<CH, true, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
         else
-            k_forces<CH, true, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax_log);
+            k_forces<CH, true, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
     } else {
         if (c.has_mhd)
-            k_forces<CH, false, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax_log);
+            k_forces<CH, false, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
         else
-            k_forces<CH, false, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax_log);
+            k_forces<CH, false, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
     }
 }
 

@@ -16,7 +16,8 @@ struct LaunchCtx {
     int has_mhd;
     int density_chunk;
     int force_chunk;
-    int kmax_log;
+    int kmax;     // max lanes per target in the pair passes
+    int balance;  // share a target's accepted pairs among its K lanes
     StateBuf S;          // current buffer set
     WorkBuf W;
     hipStream_t stream;
