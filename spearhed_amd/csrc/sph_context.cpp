@@ -35,7 +35,7 @@ struct sph_ctx {
     WorkBuf W = {};
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    sph_tuning tune = {256, 128, 0, 16, 0};
+    sph_tuning tune = {128, 64, 0, 64, 1};  // autotuned on the 2M-particle MHD turbulence step
     void* arena = nullptr;
     size_t arena_bytes = 0;
     bool binned = false;   // keys + histogram valid for the current positions

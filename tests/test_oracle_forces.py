@@ -1,0 +1,316 @@
+"""Pins the oracle's force pass, reductions and integrator (CPU only).
+
+Reference known-answer test restated: tests/spearhed/unit/HydroForces/HydroForces.cpp:159-239
+(two particles at rest at +-0.2, h 0.5, rho = u = m = 1, gamma 5/3, cubic kernel:
+dvdt_x = +-2 m P/rho^2 dW/dr(0.4, 0.5), dvdt_y = dvdt_z = 0, dudt = 0; rel 1e-4, abs 1e-6).
+
+The physics the reference lacks is pinned by results the mathematics fixes:
+  * two-body Maxwell stress: B parallel to the separation turns P into P - B^2/2, B perpendicular
+    into P + B^2/2, and the Borve correction removes the tension (P + B^2/2 again);
+  * induction in a linear shear flow v = (0, S x, 0) with B = (B0, 0, 0): dB/dt = (0, S B0, 0);
+  * divergence of B = (b x, 0, 0): div B = b; cleaning damps psi as -psi sigma c_h / h;
+  * exact pairwise conservation: sum m a = 0 and sum m (v.a + du/dt) = 0 (hydro + viscosity);
+  * Galilean invariance; a uniform lattice feels no force; viscosity acts only on approaching pairs
+    and only heats;
+  * the CFL dt of a uniform state at rest: cfl * h / (2 c).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle_ref as O
+from spearhed_amd import capi, ic
+from spearhed_amd.particles import ParticleSet
+
+
+def _set(pos, m, h, rho=1.0, u=1.0):
+    n = pos.shape[1]
+    ps = ParticleSet(n)
+    ps.pos[:] = pos
+    ps.mass[:] = m
+    ps.h[:] = h
+    ps.rho[:] = rho
+    ps.u[:] = u
+    return ps
+
+
+def _forces(cfg, ps, prec="f64", rho=None):
+    o = O.OracleRun(cfg, ps, prec, scales=True)
+    o.resort()
+    o.density()
+    if rho is not None:
+        o.rho[:] = rho
+    o.forces()
+    return o, o.by_id()
+
+
+@pytest.mark.parametrize("prec", ["plain", "f64"])
+def test_hydro_two_particle_kat(oracle_lib, prec):
+    h, d, gam = 0.5, 0.4, 5.0 / 3.0
+    cfg = capi.make_config(box_lo=(-2, -2, -2), box_len=(4, 4, 4), periodic=(0, 0, 0), kernel=capi.KERNEL_CUBIC,
+                           h0=h, gamma=gam, alpha_visc=0.0)
+    pos = np.zeros((3, 2))
+    pos[0] = [-d / 2, d / 2]
+    o, order = _forces(cfg, _set(pos, 1.0, h), prec, rho=1.0)
+    P = (gam - 1) * 1.0 * 1.0
+    q = d / h
+    dwdr = (1.0 / math.pi) / h ** 4 * (2.25 * q * q - 3 * q)
+    expected_left = 1.0 * 2 * P / 1.0 * dwdr
+    np.testing.assert_allclose(o.dvdt[0][order], [expected_left, -expected_left], rtol=1e-4)
+    np.testing.assert_allclose(o.dvdt[1:][:, order], 0.0, atol=1e-6)
+    np.testing.assert_allclose(o.dudt[order], 0.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("orient,borve,p_eff", [("x", 0, "P-B2/2"), ("y", 0, "P+B2/2"), ("x", 1, "P+B2/2")])
+def test_mhd_two_particle_stress(oracle_lib, orient, borve, p_eff):
+    h, d, gam, B0 = 0.5, 0.5, float(np.float32(5.0 / 3.0)), float(np.float32(0.7))
+    cfg = capi.make_config(box_lo=(-2, -2, -2), box_len=(4, 4, 4), periodic=(0, 0, 0), kernel=capi.KERNEL_WENDLAND,
+                           h0=h, gamma=gam, alpha_visc=0.0, mhd=1, cleaning=0, borve=borve)
+    pos = np.zeros((3, 2))
+    pos[0] = [-d / 2, d / 2]
+    ps = _set(pos, 1.0, h)
+    ps.B[0 if orient == "x" else 1] = B0
+    o, order = _forces(cfg, ps, "f64", rho=1.0)
+    P = (gam - 1) * 1.0 * 1.0
+    Peff = P - B0 ** 2 / 2 if p_eff == "P-B2/2" else P + B0 ** 2 / 2
+    # Wendland C2, H = 2h: dW/dr = -(105/(16 pi h^4)) q (1 - q/2)^3
+    q = d / h
+    dwdr = -(105.0 / (16 * math.pi * h ** 4)) * q * (1 - q / 2) ** 3
+    expected_left = 2 * 1.0 * Peff * dwdr  # same form as the hydro two-body force, with P -> P_eff
+    np.testing.assert_allclose(o.dvdt[0][order], [expected_left, -expected_left], rtol=1e-6)
+    np.testing.assert_allclose(o.dvdt[1:][:, order], 0.0, atol=1e-12)
+
+
+def _lattice_box(n, periodic, jitter_seed=None):
+    dx = 1.0 / n
+    h = float(np.float32(ic.h_for_spacing(dx)))
+    rng = np.random.default_rng(jitter_seed) if jitter_seed is not None else None
+    pos = ic.lattice(n, n, n, (0, 0, 0), dx, rng)
+    m = float(np.float32(dx ** 3))
+    lo = (0, 0, 0) if periodic else (-0.5, -0.5, -0.5)
+    L = (1, 1, 1) if periodic else (2, 2, 2)
+    return pos, m, h, lo, L
+
+
+def _interior(pos, margin):
+    return np.all((pos > margin) & (pos < 1 - margin), axis=0)
+
+
+def _lattice_gradient_factor(h, dx):
+    """Exact lattice sums in FP64: rho = m sum W(r) and g = -m sum s(r) x^2 with the Wendland C2
+    gradient scalar s = -(105/(16 pi h^5)) (1 - q/2)^3; the difference-form SPH derivative of a
+    linear field f = a x on a perfect lattice is exactly a * g / rho."""
+    m = float(np.float32(dx ** 3))
+    off = np.arange(-5, 6)
+    X, Y, Z = np.meshgrid(off, off, off, indexing="ij")
+    x = dx * X.ravel()
+    r = dx * np.sqrt(X ** 2 + Y ** 2 + Z ** 2).ravel()
+    q = r / h
+    inside = (q < 2) & (r > 0)
+    H = 2 * h
+    w0 = 21.0 / (2 * math.pi * H ** 3)
+    rho = m * (w0 + np.sum(w0 * (1 - r[inside] / H) ** 4 * (1 + 4 * r[inside] / H)))
+    s = -(105.0 / (16 * math.pi * h ** 5)) * (1 - q[inside] / 2) ** 3
+    g = -m * np.sum(s * x[inside] ** 2)
+    return g / rho
+
+
+def test_induction_linear_shear(oracle_lib):
+    """v = (0, S x, 0), B = (B0, 0, 0): dB/dt = (B.grad) v - B div v = (0, S B0, 0), which the SPH
+    operator gives on a lattice as S B0 g/rho (g/rho = 0.97 at 57 neighbours, exact lattice sum)."""
+    n = 16
+    pos, m, h, lo, L = _lattice_box(n, periodic=False)
+    cfg = capi.make_config(box_lo=lo, box_len=L, periodic=(0, 0, 0), kernel=capi.KERNEL_WENDLAND, h0=h, mhd=1,
+                           cleaning=1, alpha_visc=0.0)
+    S, B0 = float(np.float32(0.8)), float(np.float32(0.3))
+    ps = _set(pos, m, h, rho=1.0, u=1.5)
+    ps.vel[1] = S * pos[0]
+    ps.B[0] = B0
+    o, order = _forces(cfg, ps, "f64")
+    inner = _interior(pos, 2.5 * 2 * h)
+    db = o.dbdt[:, order][:, inner]
+    assert inner.sum() > 20
+    fac = _lattice_gradient_factor(h, 1.0 / n)
+    assert abs(fac - 1.0) < 0.05
+    np.testing.assert_allclose(db[1], S * B0 * fac, rtol=1e-6)
+    np.testing.assert_allclose(db[0], 0.0, atol=1e-10)
+    np.testing.assert_allclose(db[2], 0.0, atol=1e-10)
+
+
+def test_divergence_of_linear_field_and_cleaning_sign(oracle_lib):
+    """B = (b x, 0, 0): div B = b, given on a lattice as b g/rho (exact lattice sum)."""
+    n = 16
+    pos, m, h, lo, L = _lattice_box(n, periodic=False)
+    cfg = capi.make_config(box_lo=lo, box_len=L, periodic=(0, 0, 0), kernel=capi.KERNEL_WENDLAND, h0=h, mhd=1,
+                           cleaning=1, alpha_visc=0.0)
+    b = 0.25
+    ps = _set(pos, m, h, rho=1.0, u=1.5)
+    ps.B[0] = (b * pos[0]).astype(np.float32)
+    o, order = _forces(cfg, ps, "f64")
+    inner = _interior(pos, 2.5 * 2 * h)
+    fac = _lattice_gradient_factor(h, 1.0 / n)
+    np.testing.assert_allclose(o.divb[order][inner], b * fac, rtol=1e-5)
+    assert np.all(o.dpsidt[order][inner] < 0)  # psi' = -c_h^2 div B pushes psi against div B
+    assert o.c_h > 0
+
+
+def test_cleaning_damping_rate(oracle_lib):
+    """Uniform B, v, psi on a perfect lattice: div B = div v = 0, so psi' = -psi sigma c_h / h
+    with c_h = sqrt((gamma P + B^2) / rho)."""
+    n = 10
+    pos, m, h, lo, L = _lattice_box(n, periodic=True)
+    gam, sig = float(np.float32(5.0 / 3.0)), float(np.float32(0.8))
+    cfg = capi.make_config(periodic=(1, 1, 1), kernel=capi.KERNEL_WENDLAND, h0=h, gamma=gam, mhd=1, cleaning=1,
+                           clean_sigma=sig)
+    ps = _set(pos, m, h, rho=1.0, u=1.5)
+    ps.B[2] = 0.3
+    ps.psi[:] = 0.05
+    ps.vel[0] = 0.2
+    o, order = _forces(cfg, ps, "f64")
+    rho = o.rho[order]
+    P = (gam - 1) * rho * 1.5
+    ch = float(np.max(np.sqrt((gam * P + np.float32(0.3) ** 2) / rho)))
+    assert o.c_h == pytest.approx(ch, rel=1e-6)
+    np.testing.assert_allclose(o.dpsidt[order], -float(np.float32(0.05)) * sig * ch / h, rtol=1e-6)
+
+
+def _random_state(n=10, seed=3, mhd=0, borve=0, alpha=1.0):
+    dx = 1.0 / n
+    h = float(np.float32(ic.h_for_spacing(dx)))
+    rng = np.random.default_rng(seed)
+    pos = ic.lattice(n, n, n, (0, 0, 0), dx, rng, jitter=0.2)
+    ps = _set(pos, float(np.float32(dx ** 3)), h, rho=1.0, u=1.0)
+    ps.u[:] = rng.uniform(0.5, 2.0, ps.n)
+    ps.vel[:] = rng.normal(0, 0.5, size=(3, ps.n))
+    if mhd:
+        ps.B[:] = rng.normal(0, 0.3, size=(3, ps.n))
+        ps.psi[:] = rng.normal(0, 0.01, ps.n)
+    cfg = capi.make_config(periodic=(1, 1, 1), kernel=capi.KERNEL_WENDLAND, h0=h, mhd=mhd, cleaning=mhd, borve=borve,
+                           alpha_visc=alpha)
+    return cfg, ps
+
+
+@pytest.mark.parametrize("mhd", [0, 1])
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("mixed", 2e-5)])
+def test_momentum_conservation(oracle_lib, mhd, prec, tol):
+    cfg, ps = _random_state(mhd=mhd, borve=0)
+    o, order = _forces(cfg, ps, prec)
+    m = o.mass.astype(np.float64)
+    p = np.sum(m * o.dvdt.astype(np.float64), axis=1)
+    scale = np.sum(m * o.sc_a)
+    assert np.all(np.abs(p) <= tol * scale), (p, scale)
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("mixed", 2e-5)])
+def test_energy_conservation_hydro(oracle_lib, prec, tol):
+    cfg, ps = _random_state(mhd=0)
+    o, order = _forces(cfg, ps, prec)
+    m = o.mass.astype(np.float64)
+    v = o.vel.astype(np.float64)
+    a = o.dvdt.astype(np.float64)
+    rate = np.sum(m * (np.sum(v * a, axis=0) + o.dudt.astype(np.float64)))
+    scale = np.sum(m * (np.linalg.norm(v, axis=0) * o.sc_a + o.sc_du))
+    assert abs(rate) <= tol * scale
+
+
+def test_galilean_invariance(oracle_lib):
+    cfg, ps = _random_state(mhd=1, borve=1)
+    a, oa = _forces(cfg, ps, "f64")
+    ps2 = ps.copy()
+    ps2.vel += np.array([0.7, -0.3, 1.1], dtype=np.float32)[:, None]
+    b, ob = _forces(cfg, ps2, "f64")
+    for f, sc in (("dvdt", "sc_a"), ("dudt", "sc_du"), ("dbdt", "sc_db")):
+        x = getattr(a, f)[..., oa]
+        y = getattr(b, f)[..., ob]
+        s = getattr(a, sc)[oa]
+        assert np.max(np.abs(x - y) / s) < 1e-5  # float32 velocity shift rounds v_ij at 1e-7
+
+
+def test_uniform_lattice_feels_no_force(oracle_lib):
+    n = 10
+    pos, m, h, lo, L = _lattice_box(n, periodic=True)
+    cfg = capi.make_config(periodic=(1, 1, 1), kernel=capi.KERNEL_WENDLAND, h0=h, mhd=1, cleaning=1, borve=1)
+    ps = _set(pos, m, h, rho=1.0, u=1.5)
+    ps.B[:] = np.array([0.1, -0.2, 0.3], dtype=np.float32)[:, None]
+    ps.vel[:] = np.array([0.5, 0.1, -0.2], dtype=np.float32)[:, None]
+    o, order = _forces(cfg, ps, "f64")
+    assert np.max(np.abs(o.dvdt) / o.sc_a) < 1e-12
+    assert np.max(np.abs(o.dbdt)) < 1e-9
+    assert np.max(np.abs(o.dudt)) < 1e-9
+    # CFL: at rest relative to each other, v_sig = 2 c, no acceleration
+    c = math.sqrt((5 / 3 * (2 / 3) * o.rho[0] * 1.5 + 0.14) / o.rho[0])
+    red = o.reduce()
+    assert red["dt_next"] == pytest.approx(0.3 * h / (2 * c), rel=1e-6)
+
+
+def test_viscosity_acts_on_approach_and_heats(oracle_lib):
+    h = 0.5
+    pos = np.zeros((3, 2))
+    pos[0] = [-0.3, 0.3]
+    res = {}
+    for approach in (True, False):
+        for alpha in (0.0, 1.0):
+            cfg = capi.make_config(box_lo=(-2, -2, -2), box_len=(4, 4, 4), periodic=(0, 0, 0),
+                                   kernel=capi.KERNEL_WENDLAND, h0=h, alpha_visc=alpha, beta_visc=2.0)
+            ps = _set(pos, 1.0, h)
+            s = 1.0 if approach else -1.0
+            ps.vel[0] = [0.5 * s, -0.5 * s]
+            o, order = _forces(cfg, ps, "f64", rho=1.0)
+            res[(approach, alpha)] = (o.dvdt[:, order].copy(), o.dudt[order].copy())
+    a0, u0 = res[(False, 0.0)]
+    a1, u1 = res[(False, 1.0)]
+    assert np.array_equal(a0, a1) and np.array_equal(u0, u1)  # receding: no viscosity
+    a0, u0 = res[(True, 0.0)]
+    a1, u1 = res[(True, 1.0)]
+    da, du = a1 - a0, u1 - u0
+    assert np.all(du > 0)  # viscous heating
+    np.testing.assert_allclose(da[:, 0], -da[:, 1], rtol=1e-12, atol=1e-15)  # pairwise antisymmetric
+    assert da[0, 0] < 0 < da[0, 1]  # decelerates the approach
+
+
+@pytest.mark.parametrize("prec", ["plain", "mixed"])
+def test_drift_wraps_and_matches_numpy(oracle_lib, prec):
+    cfg = capi.make_config(box_lo=(0, 0, 0), box_len=(1, 1, 1), periodic=(1, 1, 1), h0=0.05)
+    n = 50
+    rng = np.random.default_rng(2)
+    ps = _set(rng.uniform(0, 1, size=(3, n)), 1.0, 0.05)
+    ps.vel[:] = rng.normal(0, 3, size=(3, n))
+    o = O.OracleRun(cfg, ps, prec)
+    dt = 0.1
+    o.drift(dt)
+    if prec == "plain":
+        x = ps.pos.astype(np.float32) + ps.vel * np.float32(dt)
+        x = np.where(x >= np.float32(1), x - np.float32(1), np.where(x < 0, x + np.float32(1), x))
+        assert np.array_equal(o.pos.astype(np.float32), x)
+    else:
+        x = ps.pos + ps.vel.astype(np.float64) * dt
+        x = np.where(x >= 1.0, x - 1.0, np.where(x < 0, x + 1.0, x))
+        assert np.array_equal(o.pos, x)
+    assert np.all((o.pos >= 0) & (o.pos <= 1))
+
+
+def test_reductions_match_fsum(oracle_lib):
+    cfg, ps = _random_state(mhd=1)
+    o, order = _forces(cfg, ps, "mixed")
+    red = o.reduce()
+    m = o.mass.astype(np.float64)
+    v = o.vel.astype(np.float64)
+    B = o.B.astype(np.float64)
+    e = m * (0.5 * np.sum(v * v, axis=0) + o.u.astype(np.float64) + 0.5 * np.sum(B * B, axis=0) / o.rho)
+    assert red["mass"] == pytest.approx(math.fsum(m), rel=1e-13)
+    assert red["energy"] == pytest.approx(math.fsum(e), rel=1e-12)
+    for a in range(3):
+        assert red["momentum"][a] == pytest.approx(math.fsum(m * v[a]), rel=1e-10, abs=1e-14)
+
+
+def test_fp32_oracle_close_to_fp64(oracle_lib):
+    cfg, ps = ic.mhd_turbulence(n=12, seed=5)
+    a, oa = _forces(cfg, ps, "mixed")
+    b, ob = _forces(cfg, ps, "f64")
+    assert np.array_equal(a.nnb[oa], b.nnb[ob]) or np.mean(a.nnb[oa] != b.nnb[ob]) < 1e-3
+    assert np.max(np.abs(a.rho[oa] - b.rho[ob]) / b.rho[ob]) < 1e-5
+    assert np.max(np.abs(a.dvdt[:, oa] - b.dvdt[:, ob]) / b.sc_a[ob]) < 1e-5
+    assert np.max(np.abs(a.dbdt[:, oa] - b.dbdt[:, ob]) / b.sc_db[ob]) < 1e-5
