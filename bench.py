@@ -41,6 +41,19 @@ DENSITY_FLOP_PER_PAIR = 23
 FORCE_BYTES_PER_PARTICLE = 80 + 44
 
 
+def measured_traffic(kernel: str, particles: int, tuning: dict):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    (each scaled by the copy-kernel calibration), when they were taken on this workload and tuning."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        rec = json.load(open(path))[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    if rec.get("particles") != particles or rec.get("tuning") != tuning:
+        return None
+    return rec["bytes_per_launch"]
+
+
 def run_main(args):
     t0 = time.perf_counter()
     cfg, ps = ic.mhd_turbulence(n=args.n, seed=3)
@@ -70,25 +83,49 @@ def run_main(args):
 
 
 def run_ensemble(args):
+    """Ragged ensemble of Kelvin-Helmholtz runs: one launch per stage for all runs, captured in a
+    hipGraph after the autotuner picked the launch shape, against the same runs stepped one at a
+    time in plain FP32 with default launch shapes and no graph."""
     runs = ic.kh_ensemble(args.ensemble_runs, seed=4)
     sizes = [p.n for _, p in runs]
+    total = sum(sizes)
+    k = args.ensemble_steps
     ctx = Context([c for c, _ in runs], sizes, precision="mixed")
     for r, (_, p) in enumerate(runs):
         ctx.upload(r, p)
     ctx.prime()
-    ctx.step(2)
+    tuning = ctx.autotune(2) if not args.no_tune_ensemble else ctx.tuning
+    ctx.prime()
+    ctx.graph_capture()
+    ctx.graph_launch(2)
     ctx.sync()
-    k = args.ensemble_steps
-    ctx.profile_begin(k)
     t1 = time.perf_counter()
+    ctx.graph_launch(k)
+    ctx.sync()
+    wall_graph = time.perf_counter() - t1
+    ctx.profile_begin(k)
     ctx.step(k)
     ctx.sync()
-    wall = time.perf_counter() - t1
     prof = ctx.profile_end()
     ctx.close()
-    total = sum(sizes)
-    return {"runs": len(runs), "particles": total, "steps": k, "p50_ms": prof["step_ms_p50"],
-            "p99_ms": prof["step_ms_p99"], "particle_steps_per_s": total * k / wall}
+    # plain FP32, one run at a time (contexts created and primed outside the timed regions)
+    k_plain = max(2, k // 2)
+    wall_plain = 0.0
+    for cfg, p in runs:
+        with Context([cfg], [p.n], precision="fp32") as c1:
+            c1.upload(0, p)
+            c1.prime()
+            c1.sync()
+            t1 = time.perf_counter()
+            c1.step(k_plain)
+            c1.sync()
+            wall_plain += time.perf_counter() - t1
+    batched = total * k / wall_graph
+    plain = total * k_plain / wall_plain
+    return {"runs": len(runs), "particles": total, "steps": k, "particle_steps_per_s": batched,
+            "p50_ms": prof["step_ms_p50"], "p99_ms": prof["step_ms_p99"], "tuning": tuning,
+            "plain_fp32_one_run_at_a_time": {"particle_steps_per_s": plain, "steps_per_run": k_plain},
+            "speedup_vs_plain": batched / plain}
 
 
 def cpu_baseline(cfg, ps, budget_s: float):
@@ -124,6 +161,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--no-tune-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the step would shard by "
@@ -159,12 +197,14 @@ def main():
         "neighbours_mean": pairs / n,
         "roofline": {
             "kernel": "k_forces (force pass)",
-            "bound": "valu",
+            "bound": "mfma",
+            "compute_path": "FP32 VALU (pair physics is no contraction, MFMA idle); priced against the "
+                            "published FP32 peak",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic("k_forces", n, m["tuning"]),
             "flop_per_launch": force_flop,
             "launch_ms": f_ms,
             "algorithmic_bytes_per_launch": FORCE_BYTES_PER_PARTICLE * n,

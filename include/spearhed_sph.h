@@ -145,6 +145,16 @@ typedef struct sph_ctx sph_ctx;
 int sph_ctx_create(const sph_run_config* runs, const int64_t* capacity, int32_t n_runs,
                    int32_t precision, void* stream, sph_ctx** out);
 int sph_ctx_destroy(sph_ctx* ctx);
+/* A context with fixed capacities and no runs, whose run set changes between steps (continuous
+ * batching for serving): max_cells bounds the summed supercells of resident runs; mhd = 1 allocates
+ * the B / psi fields. No counterpart in the reference, which sizes its buffers once at start-up. */
+int sph_pool_create(int64_t max_particles, int64_t max_cells, int32_t max_runs, int32_t precision, int32_t mhd,
+                    void* stream, sph_ctx** out);
+/* New run set = the kept runs (old indices, in the given order; their state and dt/time move with
+ * them) followed by the added runs (to be filled with sph_upload and given a first dt by
+ * sph_prime, which leaves the kept runs' dt untouched). */
+int sph_pool_set_runs(sph_ctx* ctx, const int32_t* keep, int32_t n_keep, const sph_run_config* add,
+                      const int64_t* add_sizes, int32_t n_add);
 const char* sph_last_error(const sph_ctx* ctx);
 int sph_sync(sph_ctx* ctx);
 int sph_get_grid(sph_ctx* ctx, int32_t run, sph_grid_info* out);
@@ -152,6 +162,11 @@ int sph_set_tuning(sph_ctx* ctx, const sph_tuning* t);
 int sph_get_tuning(sph_ctx* ctx, sph_tuning* out);
 /* Bytes of HBM a run of n particles needs at a precision (admission by HBM budget). */
 int64_t sph_bytes_per_run(const sph_run_config* cfg, int64_t n, int32_t precision);
+/* Supercells a run occupies in a pool (its grid or the finest one the autotuner may choose); -1 if
+ * the configuration is invalid. Host-only. */
+int64_t sph_run_cells(const sph_run_config* cfg);
+/* Device bytes sph_pool_create reserves for these capacities; -1 if invalid. Host-only. */
+int64_t sph_pool_bytes(int64_t max_particles, int64_t max_cells, int32_t max_runs, int32_t precision, int32_t mhd);
 
 /* ---- data movement (the caller owns the host arrays) ---------------------------------------- */
 int sph_upload(sph_ctx* ctx, int32_t run, const sph_host_soa* src);

@@ -167,12 +167,18 @@ PROTOTYPES = {
     "sph_ctx_create": (C.c_int, [C.POINTER(RunConfig), C.POINTER(C.c_int64), C.c_int32, C.c_int32, C.c_void_p,
                                  C.POINTER(C.c_void_p)]),
     "sph_ctx_destroy": (C.c_int, [_CTX]),
+    "sph_pool_create": (C.c_int, [C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                  C.POINTER(C.c_void_p)]),
+    "sph_pool_set_runs": (C.c_int, [_CTX, C.POINTER(C.c_int32), C.c_int32, C.POINTER(RunConfig),
+                                    C.POINTER(C.c_int64), C.c_int32]),
     "sph_last_error": (C.c_char_p, [_CTX]),
     "sph_sync": (C.c_int, [_CTX]),
     "sph_get_grid": (C.c_int, [_CTX, C.c_int32, C.POINTER(GridInfo)]),
     "sph_set_tuning": (C.c_int, [_CTX, C.POINTER(Tuning)]),
     "sph_get_tuning": (C.c_int, [_CTX, C.POINTER(Tuning)]),
     "sph_bytes_per_run": (C.c_int64, [C.POINTER(RunConfig), C.c_int64, C.c_int32]),
+    "sph_run_cells": (C.c_int64, [C.POINTER(RunConfig)]),
+    "sph_pool_bytes": (C.c_int64, [C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
     "sph_upload": (C.c_int, [_CTX, C.c_int32, C.POINTER(HostSoA)]),
     "sph_download": (C.c_int, [_CTX, C.c_int32, C.POINTER(HostSoA)]),
     "sph_stage_drift": (C.c_int, [_CTX, C.c_double]),

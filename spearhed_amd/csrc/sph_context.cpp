@@ -30,6 +30,10 @@ struct sph_ctx {
     int ncell_tot = 0;
     int ncell_cap = 0;
     int nkb = 0;
+    int64_t cap_particles = 0;
+    int cap_runs = 0;
+    int cap_kb = 0;
+    bool pool_mhd = false;
     StateBuf S[2] = {};
     int cur = 0;
     WorkBuf W = {};
@@ -195,6 +199,7 @@ void carve(Carver& cv, sph_ctx* c, int64_t N, int ncell_cap, int nkb, int nruns)
     W.partial = cv.take<double>((int64_t)nkb * 6);
     W.cons = cv.take<double>((int64_t)nruns * 8);
     W.blk_run = cv.take<int32_t>(nkb);
+    W.prime_flag = cv.take<int32_t>(nruns);
 }
 
 LaunchCtx lctx(sph_ctx* c)
@@ -304,55 +309,80 @@ int do_step(sph_ctx* c)
     return rc;
 }
 
-int alloc_ctx(sph_ctx* c, const int64_t* capacity)
+// Allocate the arena for fixed capacities (particles, supercells, runs, particle-kernel blocks).
+int reserve(sph_ctx* c, int64_t cap_p, int64_t cap_c, int cap_r, int cap_kb)
 {
-    c->runs_h.resize(c->nruns);
+    if (cap_p < 1 || cap_r < 1 || cap_c < 1) return fail(c, SPH_E_INVALID, "capacities must be >= 1");
+    if (cap_p >= (1LL << 32) - 1) return fail(c, SPH_E_CAPACITY, "more than 2^32-1 particles in one context");
+    if (cap_p >= (1LL << 27)) return fail(c, SPH_E_CAPACITY, "more than 2^27 particles in one context");
+    if (cap_c >= (1LL << 30)) return fail(c, SPH_E_CAPACITY, "too many supercells");
+    c->cap_particles = cap_p;
+    c->ncell_cap = (int)cap_c;
+    c->cap_runs = cap_r;
+    c->cap_kb = cap_kb;
+    Carver dry{nullptr, 0, true};
+    carve(dry, c, cap_p, c->ncell_cap, cap_kb, cap_r);
+    c->arena_bytes = dry.off + 256;
+    HIPTRY(c, hipMalloc(&c->arena, c->arena_bytes));
+    Carver real{reinterpret_cast<char*>(c->arena), 0, false};
+    carve(real, c, cap_p, c->ncell_cap, cap_kb, cap_r);
+    HIPTRY(c, hipMemset(c->arena, 0, c->arena_bytes));
+    HIPTRY(c, hipMalloc(&c->runs_d, sizeof(RunDev) * cap_r));
+    return SPH_OK;
+}
+
+// Lay runs out contiguously (particles, supercells, kick blocks) and upload the run and block
+// tables. Runs flagged in `fresh` get their dt reset (dt_fixed or 0) and are marked for priming.
+int layout(sph_ctx* c, const std::vector<sph_run_config>& cfg, const std::vector<int64_t>& sizes,
+           const std::vector<char>& fresh, const std::vector<double>& dt, const std::vector<double>& time)
+{
+    const int nr = (int)cfg.size();
+    if (nr > c->cap_runs) return fail(c, SPH_E_CAPACITY, "run table capacity exceeded");
+    std::vector<RunDev> runs(nr);
     int64_t p0 = 0;
-    int c0 = 0, kb = 0;
-    int64_t ncell_max_total = 0;
-    for (int r = 0; r < c->nruns; ++r) {
+    int64_t c0 = 0;
+    int kb = 0;
+    for (int r = 0; r < nr; ++r) {
         std::string why;
-        if (!make_run(c->cfg[r], c->runs_h[r], why)) return fail(c, SPH_E_INVALID, "run " + std::to_string(r) + ": " + why);
-        if (capacity[r] < 1) return fail(c, SPH_E_INVALID, "run capacity must be >= 1");
-        RunDev& R = c->runs_h[r];
+        if (!make_run(cfg[r], runs[r], why)) return fail(c, SPH_E_INVALID, "run " + std::to_string(r) + ": " + why);
+        if (sizes[r] < 1) return fail(c, SPH_E_INVALID, "run size must be >= 1");
+        RunDev& R = runs[r];
         R.p0 = p0;
-        R.np = capacity[r];
-        R.c0 = c0;
+        R.np = sizes[r];
+        R.c0 = (int32_t)c0;
         R.kb_first = kb;
         R.kb_count = (int32_t)((R.np + 255) / 256);
         p0 += R.np;
         c0 += R.ncell;
         kb += R.kb_count;
-        // capacity for the autotuner's finest grid (cell_factor 1)
-        sph_run_config fine = c->cfg[r];
-        fine.cell_factor = 1.0f;
-        RunDev Rf;
-        make_run(fine, Rf, why);
-        ncell_max_total += std::max(Rf.ncell, R.ncell);
     }
-    if (p0 >= (1LL << 32) - 1) return fail(c, SPH_E_CAPACITY, "more than 2^32-1 particles in one context");
-    if (ncell_max_total >= (1LL << 31)) return fail(c, SPH_E_CAPACITY, "too many supercells");
+    if (p0 > c->cap_particles) return fail(c, SPH_E_CAPACITY, "particle capacity exceeded");
+    if (c0 > c->ncell_cap) return fail(c, SPH_E_CAPACITY, "supercell capacity exceeded");
+    if (kb > c->cap_kb) return fail(c, SPH_E_CAPACITY, "block table capacity exceeded");
+    c->nruns = nr;
+    c->cfg = cfg;
+    c->runs_h = runs;
     c->ntot = p0;
-    c->ncell_tot = c0;
-    c->ncell_cap = (int)ncell_max_total;
+    c->ncell_tot = (int)c0;
     c->nkb = kb;
-    Carver dry{nullptr, 0, true};
-    carve(dry, c, c->ntot, c->ncell_cap, c->nkb, c->nruns);
-    c->arena_bytes = dry.off + 256;
-    HIPTRY(c, hipMalloc(&c->arena, c->arena_bytes));
-    Carver real{reinterpret_cast<char*>(c->arena), 0, false};
-    carve(real, c, c->ntot, c->ncell_cap, c->nkb, c->nruns);
-    // zero the whole arena first (histogram, accumulators), then write the tables into it
-    HIPTRY(c, hipMemset(c->arena, 0, c->arena_bytes));
-    HIPTRY(c, hipMalloc(&c->runs_d, sizeof(RunDev) * c->nruns));
-    HIPTRY(c, hipMemcpy(c->runs_d, c->runs_h.data(), sizeof(RunDev) * c->nruns, hipMemcpyHostToDevice));
-    std::vector<int32_t> blk(c->nkb);
-    for (int r = 0; r < c->nruns; ++r)
-        for (int b = 0; b < c->runs_h[r].kb_count; ++b) blk[c->runs_h[r].kb_first + b] = r;
-    HIPTRY(c, hipMemcpy(c->W.blk_run, blk.data(), sizeof(int32_t) * c->nkb, hipMemcpyHostToDevice));
-    std::vector<double> dt0(c->nruns);
-    for (int r = 0; r < c->nruns; ++r) dt0[r] = c->cfg[r].dt_fixed > 0 ? c->cfg[r].dt_fixed : 0.0;
-    HIPTRY(c, hipMemcpy(c->W.dt, dt0.data(), sizeof(double) * c->nruns, hipMemcpyHostToDevice));
+    if (nr > 0) {
+        HIPTRY(c, hipMemcpyAsync(c->runs_d, c->runs_h.data(), sizeof(RunDev) * nr, hipMemcpyHostToDevice, c->stream));
+        std::vector<int32_t> blk(kb);
+        for (int r = 0; r < nr; ++r)
+            for (int b = 0; b < runs[r].kb_count; ++b) blk[runs[r].kb_first + b] = r;
+        HIPTRY(c, hipMemcpyAsync(c->W.blk_run, blk.data(), sizeof(int32_t) * kb, hipMemcpyHostToDevice, c->stream));
+        std::vector<int32_t> flags(nr);
+        for (int r = 0; r < nr; ++r) flags[r] = fresh[r] ? 1 : 0;
+        HIPTRY(c, hipMemcpyAsync(c->W.prime_flag, flags.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice, c->stream));
+        HIPTRY(c, hipMemcpyAsync(c->W.dt, dt.data(), sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+        HIPTRY(c, hipMemcpyAsync(c->W.time, time.data(), sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+        HIPTRY(c, hipStreamSynchronize(c->stream));
+    }
+    HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_cap, c->stream));
+    c->binned = false;
+    c->sorted = false;
+    c->dens_valid = false;
+    invalidate_graph(c);
     return SPH_OK;
 }
 
@@ -404,6 +434,28 @@ int fetch_col(sph_ctx* c, T* dst_host, const T* src_dev, int64_t n)
     return SPH_OK;
 }
 
+// Supercells a run may need: its own grid or the finest (cell_factor 1) the autotuner may pick.
+int64_t cells_needed(const sph_run_config& cfg, std::string& why)
+{
+    RunDev R, Rf;
+    if (!make_run(cfg, R, why)) return -1;
+    sph_run_config fine = cfg;
+    fine.cell_factor = 1.0f;
+    make_run(fine, Rf, why);
+    return std::max<int64_t>(R.ncell, Rf.ncell);
+}
+
+int open_stream(sph_ctx* c, void* stream)
+{
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+        return SPH_OK;
+    }
+    HIPTRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = true;
+    return SPH_OK;
+}
+
 }  // namespace
 
 // =================================================================================================
@@ -418,23 +470,132 @@ int sph_ctx_create(const sph_run_config* runs, const int64_t* capacity, int32_t 
     *out = nullptr;
     if (precision != SPH_PREC_FP32 && precision != SPH_PREC_MIXED) return SPH_E_INVALID;
     sph_ctx* c = new sph_ctx();
+    *out = c;  // returned even on failure so that sph_last_error can report; caller destroys
     c->prec = precision;
     c->mixed = precision == SPH_PREC_MIXED;
-    c->nruns = n_runs;
-    c->cfg.assign(runs, runs + n_runs);
-    for (auto& r : c->cfg) c->has_mhd = c->has_mhd || r.mhd;
-    if (stream) {
-        c->stream = (hipStream_t)stream;
-    } else {
-        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-            delete c;
-            return SPH_E_DEVICE;
-        }
-        c->own_stream = true;
+    std::vector<sph_run_config> cfg(runs, runs + n_runs);
+    std::vector<int64_t> sizes(capacity, capacity + n_runs);
+    int64_t cap_p = 0, cap_c = 0;
+    int cap_kb = 0;
+    for (int r = 0; r < n_runs; ++r) {
+        std::string why;
+        const int64_t nc = cells_needed(cfg[r], why);
+        if (nc < 0) return fail(c, SPH_E_INVALID, "run " + std::to_string(r) + ": " + why);
+        if (sizes[r] < 1) return fail(c, SPH_E_INVALID, "run capacity must be >= 1");
+        cap_p += sizes[r];
+        cap_c += nc;
+        cap_kb += (int)((sizes[r] + 255) / 256);
+        c->has_mhd = c->has_mhd || cfg[r].mhd;
     }
-    int rc = alloc_ctx(c, capacity);
-    *out = c;  // returned even on failure so that sph_last_error can report; caller destroys
+    int rc = open_stream(c, stream);
+    if (!rc) rc = reserve(c, cap_p, cap_c, n_runs, cap_kb);
+    if (rc) return rc;
+    std::vector<char> fresh(n_runs, 1);
+    std::vector<double> dt(n_runs), time(n_runs, 0.0);
+    for (int r = 0; r < n_runs; ++r) dt[r] = cfg[r].dt_fixed > 0 ? cfg[r].dt_fixed : 0.0;
+    return layout(c, cfg, sizes, fresh, dt, time);
+}
+
+int sph_pool_create(int64_t max_particles, int64_t max_cells, int32_t max_runs, int32_t precision, int32_t mhd,
+                    void* stream, sph_ctx** out)
+{
+    if (!out || max_particles < 1 || max_cells < 1 || max_runs < 1) return SPH_E_INVALID;
+    *out = nullptr;
+    if (precision != SPH_PREC_FP32 && precision != SPH_PREC_MIXED) return SPH_E_INVALID;
+    sph_ctx* c = new sph_ctx();
+    *out = c;
+    c->prec = precision;
+    c->mixed = precision == SPH_PREC_MIXED;
+    c->has_mhd = mhd != 0;
+    const int64_t cap_kb = max_particles / 256 + max_runs;
+    if (cap_kb >= (1LL << 31)) return fail(c, SPH_E_CAPACITY, "too many particle blocks");
+    int rc = open_stream(c, stream);
+    if (!rc) rc = reserve(c, max_particles, max_cells, max_runs, (int)cap_kb);
     return rc;
+}
+
+int sph_pool_set_runs(sph_ctx* c, const int32_t* keep, int32_t n_keep, const sph_run_config* add,
+                      const int64_t* add_sizes, int32_t n_add)
+{
+    if (!c || n_keep < 0 || n_add < 0 || (n_keep && !keep) || (n_add && (!add || !add_sizes))) return SPH_E_INVALID;
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    std::vector<char> seen(c->nruns, 0);
+    for (int k = 0; k < n_keep; ++k) {
+        if (keep[k] < 0 || keep[k] >= c->nruns || seen[keep[k]])
+            return fail(c, SPH_E_INVALID, "keep list must name distinct existing runs");
+        seen[keep[k]] = 1;
+    }
+    std::vector<sph_run_config> cfg;
+    std::vector<int64_t> sizes;
+    std::vector<char> fresh;
+    for (int k = 0; k < n_keep; ++k) {
+        cfg.push_back(c->cfg[keep[k]]);
+        sizes.push_back(c->runs_h[keep[k]].np);
+        fresh.push_back(0);
+    }
+    int64_t cells = 0, parts = 0;
+    for (int k = 0; k < n_keep; ++k) {
+        std::string why;
+        cells += cells_needed(c->cfg[keep[k]], why);
+        parts += c->runs_h[keep[k]].np;
+    }
+    for (int a = 0; a < n_add; ++a) {
+        std::string why;
+        const int64_t nc = cells_needed(add[a], why);
+        if (nc < 0) return fail(c, SPH_E_INVALID, "added run " + std::to_string(a) + ": " + why);
+        if (add[a].mhd && !c->has_mhd) return fail(c, SPH_E_INVALID, "pool was created without MHD fields");
+        if (add_sizes[a] < 1) return fail(c, SPH_E_INVALID, "run size must be >= 1");
+        cfg.push_back(add[a]);
+        sizes.push_back(add_sizes[a]);
+        fresh.push_back(1);
+        cells += nc;
+        parts += add_sizes[a];
+    }
+    if (parts > c->cap_particles || cells > c->ncell_cap || (int)cfg.size() > c->cap_runs)
+        return fail(c, SPH_E_CAPACITY, "runs do not fit the pool's capacity");
+    // per-run scalars of the kept runs
+    std::vector<double> old_dt(c->nruns), old_time(c->nruns);
+    if (c->nruns) {
+        HIPTRY(c, hipMemcpy(old_dt.data(), c->W.dt, sizeof(double) * c->nruns, hipMemcpyDeviceToHost));
+        HIPTRY(c, hipMemcpy(old_time.data(), c->W.time, sizeof(double) * c->nruns, hipMemcpyDeviceToHost));
+    }
+    std::vector<double> dt, time;
+    for (int k = 0; k < n_keep; ++k) {
+        dt.push_back(old_dt[keep[k]]);
+        time.push_back(old_time[keep[k]]);
+    }
+    for (int a = 0; a < n_add; ++a) {
+        dt.push_back(add[a].dt_fixed > 0 ? add[a].dt_fixed : 0.0);
+        time.push_back(0.0);
+    }
+    // move the kept runs' state into the other buffer set at their new offsets
+    const StateBuf& A = c->S[c->cur];
+    const StateBuf& B = c->S[c->cur ^ 1];
+    int64_t dst = 0;
+    for (int k = 0; k < n_keep; ++k) {
+        const int64_t src = c->runs_h[keep[k]].p0, n = c->runs_h[keep[k]].np;
+        auto mv = [&](auto* to, auto* from) -> hipError_t {
+            if (!to || !from) return hipSuccess;
+            return hipMemcpyAsync(to + dst, from + src, sizeof(*from) * n, hipMemcpyDeviceToDevice, c->stream);
+        };
+        HIPTRY(c, mv(B.id, A.id));
+        HIPTRY(c, mv(B.mask, A.mask));
+        HIPTRY(c, mv(B.species, A.species));
+        for (int a = 0; a < 3; ++a) {
+            HIPTRY(c, mv(B.pos64[a], A.pos64[a]));
+            HIPTRY(c, mv(B.pos32[a], A.pos32[a]));
+            HIPTRY(c, mv(B.vel[a], A.vel[a]));
+            HIPTRY(c, mv(B.B[a], A.B[a]));
+        }
+        HIPTRY(c, mv(B.mass, A.mass));
+        HIPTRY(c, mv(B.h, A.h));
+        HIPTRY(c, mv(B.u, A.u));
+        HIPTRY(c, mv(B.rho, A.rho));
+        HIPTRY(c, mv(B.psi, A.psi));
+        dst += n;
+    }
+    c->cur ^= 1;
+    return layout(c, cfg, sizes, fresh, dt, time);
 }
 
 int sph_ctx_destroy(sph_ctx* c)
@@ -510,6 +671,25 @@ int64_t sph_bytes_per_run(const sph_run_config* cfg, int64_t n, int32_t precisio
     return n * (2 * state + work) + (int64_t)R.ncell * (12 + 4 * kMaskWords * 64) + 4096;
 }
 
+int64_t sph_pool_bytes(int64_t max_particles, int64_t max_cells, int32_t max_runs, int32_t precision, int32_t mhd)
+{
+    if (max_particles < 1 || max_cells < 1 || max_runs < 1 || max_cells >= (1LL << 30)) return -1;
+    sph_ctx tmp;
+    tmp.prec = precision;
+    tmp.mixed = precision == SPH_PREC_MIXED;
+    tmp.has_mhd = mhd != 0;
+    Carver dry{nullptr, 0, true};
+    carve(dry, &tmp, max_particles, (int)max_cells, (int)(max_particles / 256 + max_runs), max_runs);
+    return (int64_t)dry.off + 256 + (int64_t)sizeof(RunDev) * max_runs;
+}
+
+int64_t sph_run_cells(const sph_run_config* cfg)
+{
+    if (!cfg) return -1;
+    std::string why;
+    return cells_needed(*cfg, why);
+}
+
 int sph_upload(sph_ctx* c, int32_t run, const sph_host_soa* s)
 {
     if (!c || !s || run < 0 || run >= c->nruns) return SPH_E_INVALID;
@@ -552,6 +732,12 @@ int sph_upload(sph_ctx* c, int32_t run, const sph_host_soa* s)
     rc |= copy_col<float>(c, S.rho + o, s->density, n, 1.0f);
     if (c->has_mhd) rc |= copy_col<float>(c, S.psi + o, s->psi, n, 0.0f);
     if (rc) return SPH_E_DEVICE;
+    // new initial data: the run restarts at t = 0 and takes its first dt from the next prime
+    const double dt0 = c->cfg[run].dt_fixed > 0 ? c->cfg[run].dt_fixed : 0.0, t0 = 0.0;
+    const int32_t one = 1;
+    HIPTRY(c, hipMemcpyAsync(c->W.dt + run, &dt0, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, hipMemcpyAsync(c->W.time + run, &t0, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, hipMemcpyAsync(c->W.prime_flag + run, &one, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     HIPTRY(c, hipStreamSynchronize(c->stream));
     if (c->binned) HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_tot, c->stream));
     c->binned = false;
@@ -682,16 +868,19 @@ int sph_reduce_conserved(sph_ctx* c, sph_conserved* o)
 int sph_prime(sph_ctx* c)
 {
     if (!c) return SPH_E_INVALID;
+    if (c->nruns == 0) return SPH_OK;
     int rc = do_resort(c);
     if (!rc) rc = do_density(c);
     if (!rc) rc = do_forces(c);
-    if (!rc) rc = do_kick(c, 0.0, 0);
+    if (!rc) rc = do_kick(c, 0.0, 0);  // sets the first dt of runs flagged by upload / set_runs
+    if (!rc) HIPTRY(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->W.prime_flag), 0u, c->nruns, c->stream));
     return rc;
 }
 
 int sph_step(sph_ctx* c, int32_t n_steps)
 {
     if (!c || n_steps < 0) return SPH_E_INVALID;
+    if (c->nruns == 0) return SPH_OK;
     for (int s = 0; s < n_steps; ++s) {
         int rc = do_step(c);
         if (rc) return rc;

@@ -83,6 +83,7 @@ struct WorkBuf {
     double* partial;  // per kick block: 6 doubles (dt, M, Px, Py, Pz, E)
     double* cons;     // per run: 8 doubles (mass, px, py, pz, energy, dt_next, time, c_h)
     int32_t* blk_run; // per particle-kernel block: run id
+    int32_t* prime_flag; // per run: 1 = the next non-advancing reduction sets the run's first dt
 };
 
 constexpr float kSentinel = 8.507059e37f;  // FLT_MAX / 4: sentinel^2 overflows to +inf, so r2 < H2 fails

@@ -1008,7 +1008,9 @@ __global__ __launch_bounds__(256) void k_finalize(const RunDev* __restrict__ run
         c[5] = dtn;
         c[6] = t;
         c[7] = (double)__uint_as_float(W.chbits[r]);
-        W.dt[r] = dtn;
+        // a step sets the next dt; a reduction outside a step (prime, queries) sets it only for
+        // runs still waiting for their first dt, so runs already stepping keep their own sequence
+        if (advance || W.prime_flag[r]) W.dt[r] = dtn;
     }
 }
 

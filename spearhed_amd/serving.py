@@ -1,0 +1,205 @@
+"""Continuous-batching server of independent SPH runs on one GPU.
+
+Requests (a run configuration, its initial particles and a number of steps) arrive over time and
+wait in a FIFO queue. The server keeps one device pool (sph_pool_create) sized from an HBM budget;
+before every batched step it retires runs that have finished their steps (their final state is
+downloaded) and admits waiting requests in arrival order while they fit the budget:
+
+  * the HBM cost of a run is sph_bytes_per_run(cfg, n) (state, work arrays, supercell tables);
+  * the pool's particle / supercell / run capacities bound the resident set as well.
+
+All resident runs then advance one step together (one launch per stage for the whole batch). Every
+run keeps its own dt and time, and its summation orders depend only on its own data, so a run's
+result is bit-identical whether it is served alone or in a mix.
+
+The step latency recorded per batched step is the wall time of the step (host launch to device
+completion); p50 / p99 come from those samples.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import heapq
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import capi
+from .capi import RunConfig
+from .particles import ParticleSet
+from .stepper import Pool
+
+
+def run_bytes(cfg: RunConfig, n: int, precision: str = "mixed") -> int:
+    """Device bytes a run of n particles costs in the admission budget (sph_bytes_per_run)."""
+    lib = capi.load()
+    b = int(lib.sph_bytes_per_run(C.byref(cfg), int(n), capi.PREC_MIXED if precision == "mixed" else capi.PREC_FP32))
+    if b < 0:
+        raise capi.SphError("invalid run configuration")
+    return b
+
+
+@dataclass
+class Request:
+    rid: int
+    cfg: RunConfig
+    ps: ParticleSet
+    steps: int
+    arrival: float = 0.0  # seconds after the server's start
+
+
+@dataclass
+class Result:
+    rid: int
+    ps: ParticleSet
+    conserved: dict
+    steps: int
+    arrival: float
+    admitted: float
+    finished: float
+
+
+@dataclass
+class _Resident:
+    req: Request
+    done: int = 0
+    admitted: float = 0.0
+
+
+@dataclass
+class ServeStats:
+    step_ms: List[float] = field(default_factory=list)
+    batch_runs: List[int] = field(default_factory=list)
+    batch_particles: List[int] = field(default_factory=list)
+    admit_ms: List[float] = field(default_factory=list)
+
+    def summary(self) -> dict:
+        s = np.array(self.step_ms) if self.step_ms else np.zeros(1)
+        ps = np.array(self.batch_particles, dtype=np.float64) if self.batch_particles else np.zeros(1)
+        busy = s.sum() * 1e-3
+        return {
+            "steps": len(self.step_ms),
+            "step_ms_p50": float(np.percentile(s, 50)),
+            "step_ms_p99": float(np.percentile(s, 99)),
+            "mean_batch_runs": float(np.mean(self.batch_runs)) if self.batch_runs else 0.0,
+            "particle_steps_per_s": float(ps.sum() / busy) if busy > 0 else 0.0,
+            "admission_ms_total": float(np.sum(self.admit_ms)) if self.admit_ms else 0.0,
+        }
+
+
+class Server:
+    """Continuous batching over a fixed device pool.
+
+    hbm_budget: device bytes the resident runs may use (sum of sph_bytes_per_run).
+    cells_per_particle: supercells reserved per budgeted particle (a run with ~57 neighbours uses
+        about 1/14 supercells per particle; the default leaves room for sparser boxes).
+    """
+
+    def __init__(self, hbm_budget: int, precision: str = "mixed", max_runs: int = 512, mhd: bool = True,
+                 cells_per_particle: float = 0.25, pool_factory=Pool):
+        self.lib = capi.load()
+        self.precision = precision
+        self.prec_code = capi.PREC_MIXED if precision == "mixed" else capi.PREC_FP32
+        self.budget = int(hbm_budget)
+        # largest particle capacity whose run cost fits the budget: a run's bytes grow by at least
+        # the bytes of one particle of the lightest configuration, so budget / that is an upper bound
+        probe = capi.make_config(h0=0.05, mhd=int(mhd))
+        per_p = max(1, self.lib.sph_bytes_per_run(C.byref(probe), 1 << 20, self.prec_code) // (1 << 20))
+        self.max_particles = max(1024, self.budget // per_p)
+        self.max_cells = max(64, int(self.max_particles * cells_per_particle))
+        self.max_runs = int(max_runs)
+        self.pool = pool_factory(self.max_particles, self.max_cells, self.max_runs, precision, mhd)
+        self.pool_bytes = int(self.lib.sph_pool_bytes(self.max_particles, self.max_cells, self.max_runs,
+                                                      self.prec_code, int(mhd)))
+        self.resident: List[_Resident] = []
+        self.queue: List[tuple] = []  # (arrival, rid, Request)
+        self.stats = ServeStats()
+        self.results: dict = {}
+        self._t0 = None
+
+    # -- bookkeeping -------------------------------------------------------------------------------
+    def run_bytes(self, req: Request) -> int:
+        return run_bytes(req.cfg, req.ps.n, self.precision)
+
+    def resident_bytes(self) -> int:
+        return sum(self.run_bytes(r.req) for r in self.resident)
+
+    def submit(self, req: Request):
+        if (self.run_bytes(req) > self.budget or req.ps.n > self.max_particles
+                or self.pool.cells_needed(req.cfg) > self.max_cells):
+            raise capi.SphError(f"request {req.rid} does not fit the server's HBM budget")
+        heapq.heappush(self.queue, (req.arrival, req.rid, req))
+
+    def now(self) -> float:
+        return time.perf_counter() - self._t0
+
+    # -- one scheduling round ----------------------------------------------------------------------
+    def _retire_and_admit(self, admit_until: float):
+        """Retire finished runs, then admit queued requests that arrived by admit_until."""
+        now = self.now()
+        keep, finished = [], []
+        for i, r in enumerate(self.resident):
+            (finished if r.done >= r.req.steps else keep).append(i)
+        cons = self.pool.conserved() if finished else []
+        for i in finished:
+            r = self.resident[i]
+            self.results[r.req.rid] = Result(r.req.rid, self.pool.download(i), cons[i], r.done, r.req.arrival,
+                                             r.admitted, now)
+        used_b = sum(self.run_bytes(self.resident[i].req) for i in keep)
+        used_p = sum(self.resident[i].req.ps.n for i in keep)
+        used_c = sum(self.pool.cells_needed(self.resident[i].req.cfg) for i in keep)
+        add: List[Request] = []
+        while self.queue and self.queue[0][0] <= admit_until and len(keep) + len(add) < self.max_runs:
+            req = self.queue[0][2]
+            b, n, c = self.run_bytes(req), req.ps.n, self.pool.cells_needed(req.cfg)
+            if used_b + b > self.budget or used_p + n > self.max_particles or used_c + c > self.max_cells:
+                break  # FIFO: the head waits until enough retires
+            heapq.heappop(self.queue)
+            add.append(req)
+            used_b, used_p, used_c = used_b + b, used_p + n, used_c + c
+        if not finished and not add:
+            return
+        new_idx = self.pool.set_runs(keep, [(q.cfg, q.ps.n) for q in add])
+        self.resident = [self.resident[i] for i in keep] + [_Resident(q, 0, now) for q in add]
+        for j, q in zip(new_idx, add):
+            self.pool.upload(j, q.ps)
+        if add:
+            self.pool.prime()
+
+    def serve(self, requests: Optional[List[Request]] = None, max_steps: int = 1 << 30,
+              realtime: bool = True) -> dict:
+        """Run until every request has finished. realtime=False admits a request as soon as the
+        queue reaches it (arrival times only order the queue)."""
+        for q in requests or []:
+            self.submit(q)
+        self._t0 = time.perf_counter()
+        n_steps = 0
+        while (self.queue or self.resident) and n_steps < max_steps:
+            t_adm = time.perf_counter()
+            self._retire_and_admit(self.now() if realtime else float("inf"))
+            self.stats.admit_ms.append((time.perf_counter() - t_adm) * 1e3)
+            if not self.resident:
+                if self.queue:  # idle until the next arrival
+                    time.sleep(max(0.0, min(self.queue[0][0] - self.now(), 0.01)))
+                continue
+            t1 = time.perf_counter()
+            self.pool.step(1)
+            self.pool.sync()
+            self.stats.step_ms.append((time.perf_counter() - t1) * 1e3)
+            self.stats.batch_runs.append(len(self.resident))
+            self.stats.batch_particles.append(sum(r.req.ps.n for r in self.resident))
+            for r in self.resident:
+                r.done += 1
+            n_steps += 1
+        self._retire_and_admit(float("-inf"))  # retire the last runs, admit nothing
+        return self.stats.summary()
+
+    def close(self):
+        self.pool.close()
+
+
+def poisson_arrivals(n: int, rate_per_s: float, seed: int = 0) -> np.ndarray:
+    """Arrival times of a Poisson process (exponential gaps) with the given mean rate."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate_per_s, size=n))

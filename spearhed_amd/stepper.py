@@ -158,6 +158,50 @@ class Context:
         return t.as_dict()
 
 
+class Pool(Context):
+    """A context of fixed capacity whose run set changes between steps (continuous batching).
+
+    Capacities are set once (particles, supercells, runs); set_runs keeps a subset of the resident
+    runs, whose state and dt/time move with them, and appends new runs that the caller uploads and
+    primes. No device memory is allocated after construction.
+    """
+
+    def __init__(self, max_particles: int, max_cells: int, max_runs: int, precision: str = "mixed",
+                 mhd: bool = True):
+        self.lib = capi.load()
+        self.precision = capi.PREC_MIXED if precision == "mixed" else capi.PREC_FP32
+        self.max_particles, self.max_cells, self.max_runs = int(max_particles), int(max_cells), int(max_runs)
+        self.nruns = 0
+        self.sizes: list = []
+        self.configs: list = []
+        h = C.c_void_p()
+        rc = self.lib.sph_pool_create(self.max_particles, self.max_cells, self.max_runs, self.precision,
+                                      int(bool(mhd)), None, C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.lib.sph_last_error(h).decode() if h else ""
+            self.close()
+            raise capi.SphError(f"sph_pool_create: {capi.ERRORS.get(rc, rc)} {msg}")
+
+    def set_runs(self, keep: Sequence[int], add: Sequence[tuple]):
+        """keep: old run indices, in their new order; add: (RunConfig, n_particles) pairs appended after
+        them. Returns the new index of every added run."""
+        keep = [int(k) for k in keep]
+        k_arr = (C.c_int32 * max(1, len(keep)))(*keep)
+        a_cfg = (RunConfig * max(1, len(add)))(*[c for c, _ in add])
+        a_n = (C.c_int64 * max(1, len(add)))(*[int(n) for _, n in add])
+        check(self.lib.sph_pool_set_runs(self.h, k_arr, len(keep), a_cfg, a_n, len(add)), self.h,
+              "sph_pool_set_runs")
+        self.configs = [self.configs[k] for k in keep] + [c for c, _ in add]
+        self.sizes = [self.sizes[k] for k in keep] + [int(n) for _, n in add]
+        self.nruns = len(self.sizes)
+        return list(range(len(keep), self.nruns))
+
+    def cells_needed(self, cfg: RunConfig) -> int:
+        """Supercells a run of this configuration occupies in the pool (host-side grid arithmetic)."""
+        return int(self.lib.sph_run_cells(C.byref(cfg)))
+
+
 # -- reference-named stage functors ---------------------------------------------------------------
 class PushVelocity:
     """r += v dt for Movable species (include/spearhed/particles/pusher/PushVelocity.hpp:35-42)."""

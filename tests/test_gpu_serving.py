@@ -1,0 +1,91 @@
+"""GPU: the serving pool (sph_pool_create / sph_pool_set_runs) and the continuous-batching server.
+
+Contract: a run's results are bit-identical whether it runs alone in its own context, in a pool
+whose run set changes around it (kept runs move to new offsets, new runs join and are primed), or
+under the server's admission and retirement."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from spearhed_amd import ic
+from spearhed_amd.serving import Request, Server, run_bytes
+from spearhed_amd.stepper import Context, Pool
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("pos", "vel", "u", "rho", "B", "psi", "dvdt", "dbdt", "nnb")
+
+
+def _alone(cfg, ps, steps):
+    with Context([cfg], [ps.n], precision="mixed") as ctx:
+        ctx.upload(0, ps)
+        ctx.prime()
+        if steps:
+            ctx.step(steps)
+        ctx.sync()
+        return ctx.download(0).sorted_by_id(), ctx.conserved()[0]
+
+
+def _same(a, b):
+    for f in FIELDS:
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+def test_pool_runs_move_and_join():
+    runs = [ic.kelvin_helmholtz(nt, 500 + r) for r, nt in enumerate([3000, 6000, 4100])]
+    late = ic.orszag_tang_n(3500, 7)
+    print("pool: runs", [p.n for _, p in runs], "late", late[1].n, flush=True)
+    with Pool(40000, 4000, 8, "mixed", True) as pool:
+        pool.set_runs([], [(c, p.n) for c, p in runs])
+        for r, (_, p) in enumerate(runs):
+            pool.upload(r, p)
+        pool.prime()
+        pool.step(2)
+        # drop run 1, reverse the others, add one
+        new = pool.set_runs([2, 0], [(late[0], late[1].n)])
+        assert new == [2]
+        pool.upload(2, late[1])
+        pool.prime()
+        pool.step(3)
+        pool.sync()
+        got = [pool.download(r).sorted_by_id() for r in range(3)]
+        cons = pool.conserved()
+    for (cfg, ps), steps, g, c in zip([runs[2], runs[0], late], [5, 5, 3], got, cons):
+        a, ca = _alone(cfg, ps, steps)
+        _same(a, g)
+        assert ca["time"] == c["time"] and ca["dt_next"] == c["dt_next"]
+
+
+def test_pool_rejects_over_capacity():
+    cfg, ps = ic.kelvin_helmholtz(3000, 1)
+    with Pool(ps.n + 10, 4000, 4, "mixed", True) as pool:
+        pool.set_runs([], [(cfg, ps.n)])
+        with pytest.raises(Exception):
+            pool.set_runs([0], [(cfg, ps.n)])
+        pool.upload(0, ps)  # the failed call left the pool as it was
+        pool.prime()
+        pool.step(1)
+        pool.sync()
+        g = pool.download(0).sorted_by_id()
+    _same(_alone(cfg, ps, 1)[0], g)
+
+
+def test_served_results_equal_alone():
+    probs = [ic.kelvin_helmholtz(2500, 40), ic.brio_wu(2000, 41), ic.rotor(3000, 42),
+             ic.kelvin_helmholtz(5000, 43), ic.orszag_tang_n(2500, 44)]
+    steps = [3, 5, 2, 4, 3]
+    reqs = [Request(r, c, p, steps[r], arrival=float(r)) for r, (c, p) in enumerate(probs)]
+    budget = 3 * max(run_bytes(q.cfg, q.ps.n) for q in reqs)  # forces retire/admit cycles
+    srv = Server(budget, precision="mixed", max_runs=8)
+    try:
+        summary = srv.serve(reqs, realtime=False)
+    finally:
+        srv.close()
+    print("serve:", summary, flush=True)
+    assert sorted(srv.results) == list(range(len(reqs)))
+    for q in reqs:
+        a, ca = _alone(q.cfg, q.ps, q.steps)
+        res = srv.results[q.rid]
+        _same(a, res.ps.sorted_by_id())
+        assert res.conserved["time"] == ca["time"]

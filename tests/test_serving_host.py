@@ -1,0 +1,103 @@
+"""Host-side logic of the continuous-batching server (CPU): FIFO admission under the HBM budget,
+retirement after each request's steps, and the run moves handed to the pool. The device pool is
+replaced by a recorder; the GPU tests cover the pool itself."""
+from __future__ import annotations
+
+import pytest
+
+from spearhed_amd import capi, ic
+from spearhed_amd.serving import Request, Server, poisson_arrivals, run_bytes
+
+
+class RecorderPool:
+    """Stands in for stepper.Pool: records the calls the server makes and keeps the run table."""
+
+    def __init__(self, max_particles, max_cells, max_runs, precision, mhd):
+        self.lib = capi.load()
+        self.sizes, self.calls, self.steps = [], [], 0
+        self.max_particles, self.max_cells, self.max_runs = max_particles, max_cells, max_runs
+
+    def cells_needed(self, cfg):
+        import ctypes as C
+        return int(self.lib.sph_run_cells(C.byref(cfg)))
+
+    def set_runs(self, keep, add):
+        assert len(set(keep)) == len(keep) and all(0 <= k < len(self.sizes) for k in keep)
+        self.sizes = [self.sizes[k] for k in keep] + [n for _, n in add]
+        assert sum(self.sizes) <= self.max_particles and len(self.sizes) <= self.max_runs
+        self.calls.append(("set_runs", list(keep), len(add)))
+        return list(range(len(keep), len(self.sizes)))
+
+    def upload(self, run, ps):
+        assert ps.n == self.sizes[run]
+        self.calls.append(("upload", run))
+
+    def prime(self):
+        self.calls.append(("prime",))
+
+    def step(self, n):
+        self.steps += n
+
+    def sync(self):
+        pass
+
+    def download(self, run):
+        return ("state", run, self.sizes[run])
+
+    def conserved(self):
+        return [{"run": r} for r in range(len(self.sizes))]
+
+    def close(self):
+        pass
+
+
+def _requests(k, steps, n_target=2000):
+    out = []
+    for r in range(k):
+        cfg, ps = ic.kelvin_helmholtz(n_target, 300 + r)
+        out.append(Request(r, cfg, ps, steps[r % len(steps)], arrival=float(r)))
+    return out
+
+
+def test_budget_limits_the_resident_set():
+    reqs = _requests(6, [3, 5])
+    one = run_bytes(reqs[0].cfg, reqs[0].ps.n)
+    srv = Server(int(2.5 * one), pool_factory=RecorderPool)
+    res_sizes = []
+    orig = srv._retire_and_admit
+
+    def spy(until):
+        orig(until)
+        res_sizes.append(len(srv.resident))
+        assert srv.resident_bytes() <= srv.budget
+
+    srv._retire_and_admit = spy
+    srv.serve(reqs, realtime=False)
+    assert max(res_sizes) == 2  # two runs fit the budget, a third does not
+    assert sorted(srv.results) == list(range(6))
+    for q in reqs:
+        assert srv.results[q.rid].steps == q.steps
+
+
+def test_fifo_and_retire_order():
+    reqs = _requests(4, [2, 4])
+    srv = Server(1 << 34, pool_factory=RecorderPool)
+    srv.serve(reqs, realtime=False)
+    admits = [c for c in srv.pool.calls if c[0] == "set_runs"]
+    assert admits[0] == ("set_runs", [], 4)           # all fit: admitted together
+    assert admits[1] == ("set_runs", [1, 3], 0)       # after 2 steps runs 0 and 2 retire
+    assert srv.pool.steps == 4
+    assert srv.results[0].ps[1] == 0 and srv.results[2].ps[1] == 2  # downloaded before the move
+
+
+def test_request_larger_than_budget_is_rejected():
+    reqs = _requests(1, [1])
+    srv = Server(run_bytes(reqs[0].cfg, reqs[0].ps.n) - 1, pool_factory=RecorderPool)
+    with pytest.raises(capi.SphError):
+        srv.submit(reqs[0])
+
+
+def test_poisson_arrivals_rate():
+    t = poisson_arrivals(20000, 50.0, seed=1)
+    assert (t[1:] >= t[:-1]).all()
+    assert abs(len(t) / t[-1] - 50.0) < 2.0
