@@ -1,0 +1,93 @@
+"""The oracle against the committed golden fixtures (CPU).
+
+kernels_closed_form.npz holds closed-form kernel values computed without the oracle: the oracle's
+FP64 kernels must reproduce them to round-off and its FP32 kernels to FP32 accuracy. The <case>.npz
+fixtures hold FP64-oracle outputs on seeded small runs: the FP64 oracle must keep reproducing them
+(regression pin), and the FP32 oracle paths must stay within the parity tolerance of them.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle_ref as O
+from golden_io import CASE_NAMES, load_case, load_kernels
+from parity import RTOL, RTOL_GLOBAL
+
+
+def test_oracle_kernels_match_closed_form_fixture(oracle_lib):
+    z = load_kernels()
+    for key in z.files:
+        if not key.endswith("_r"):
+            continue
+        stem = key[:-2]
+        kernel = int(stem.split("_")[0][1:])
+        h = float(z[f"{stem}_h"])
+        for real, tol in (("f64", 1e-13), ("f32", 3e-6)):
+            w = np.array([O.kernel_w(kernel, r, h, real) for r in z[key]])
+            dw = np.array([O.kernel_dwdr(kernel, r, h, real) for r in z[key]])
+            ref_w, ref_dw = z[f"{stem}_w"], z[f"{stem}_dwdr"]
+            sw = np.max(np.abs(ref_w))
+            sd = np.max(np.abs(ref_dw))
+            assert np.max(np.abs(w - ref_w)) <= tol * sw, (stem, real)
+            assert np.max(np.abs(dw - ref_dw)) <= tol * sd, (stem, real)
+
+
+def _stages(cfg, ps, prec):
+    o = O.OracleRun(cfg, ps, prec, scales=False)
+    cs = o.resort()
+    o.density()
+    o.forces()
+    return o, cs
+
+
+@pytest.mark.parametrize("name", CASE_NAMES)
+def test_fp64_oracle_reproduces_fixture(oracle_lib, name):
+    cfg, ps, z = load_case(name)
+    o, cs = _stages(cfg, ps, "f64")
+    order = o.by_id()
+    assert np.array_equal(cs, z["f64_cell_start"])
+    assert np.array_equal(o.nnb[order], z["f64_nnb"])
+    assert np.allclose(o.rho[order], z["f64_rho"], rtol=1e-12, atol=0)
+    for f, sc in (("dvdt", "sc_a"), ("dbdt", "sc_db")):
+        assert np.all(np.abs(getattr(o, f)[:, order] - z[f"f64_{f}"]) <= 1e-12 * z[f"f64_{sc}"] + 1e-300), f
+    for f, sc in (("dudt", "sc_du"), ("divb", "sc_divb"), ("dpsidt", "sc_dpsi")):
+        assert np.all(np.abs(getattr(o, f)[order] - z[f"f64_{f}"]) <= 1e-12 * z[f"f64_{sc}"] + 1e-300), f
+
+
+# The plain path holds absolute FP32 positions, so its inputs already differ from the FP64 run by
+# the FP32 rounding of each coordinate (2^-24 of the box length, ~1e-6 of a pair separation); it is
+# held to 5e-5 against FP64. The mixed path (FP64 positions, FP32 offsets within a supercell) is
+# held to the 1e-5 contract.
+PREC_TOL = {"plain": 5e-5, "mixed": RTOL}
+
+
+@pytest.mark.parametrize("name", CASE_NAMES)
+@pytest.mark.parametrize("prec", ["plain", "mixed"])
+def test_fp32_oracle_within_tolerance_of_fixture(oracle_lib, name, prec):
+    cfg, ps, z = load_case(name)
+    o, _ = _stages(cfg, ps, prec)
+    order = o.by_id()
+    tgt = (z["species"] == 0) & (z["mask"] == 1)
+    tol = PREC_TOL[prec]
+    assert np.array_equal(O.bin_keys(cfg, prec, ps.pos), z[f"{prec}_key_by_id"])
+    assert np.array_equal(o.nnb[order], z[f"{prec}_nnb"])
+    e = np.abs(o.rho[order][tgt] - z["f64_rho"][tgt]) / z["f64_sc_rho"][tgt]
+    assert e.max() <= tol
+    e = np.abs(o.dvdt[:, order][:, tgt] - z["f64_dvdt"][:, tgt]) / z["f64_sc_a"][tgt]
+    assert e.max() <= tol
+    e = np.abs(o.dudt[order][tgt] - z["f64_dudt"][tgt]) / (z["f64_sc_du"][tgt] + 1e-300)
+    assert e.max() <= tol
+
+
+@pytest.mark.parametrize("name", CASE_NAMES)
+def test_fp32_oracle_steps_conserve_like_fixture(oracle_lib, name):
+    cfg, ps, z = load_case(name)
+    o = O.OracleRun(cfg, ps, "mixed")
+    cons = [o.prime()] + [o.step(1) for _ in range(3)]
+    m = np.array([c["mass"] for c in cons])
+    assert np.allclose(m, z["steps_mass"], rtol=1e-12, atol=0)
+    e = np.array([c["energy"] for c in cons])
+    assert np.allclose(e, z["steps_energy"], rtol=RTOL_GLOBAL, atol=0)
+    t = np.array([c["time"] for c in cons])
+    assert np.allclose(t, z["steps_time"], rtol=1e-5, atol=0)
