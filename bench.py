@@ -107,7 +107,11 @@ def run_ensemble(args):
     ctx.step(k)
     ctx.sync()
     prof = ctx.profile_end()
+    nb_mean = sum(ctx.neighbour_total(r) for r in range(len(runs))) / total
+    energy = [c["energy"] for c in ctx.conserved()]
     ctx.close()
+    if not (all(math.isfinite(e) for e in energy) and nb_mean > 10):
+        raise RuntimeError(f"ensemble state is not physical (mean neighbours {nb_mean})")
     # plain FP32, one run at a time (contexts created and primed outside the timed regions)
     k_plain = max(2, k // 2)
     wall_plain = 0.0
@@ -122,7 +126,8 @@ def run_ensemble(args):
             wall_plain += time.perf_counter() - t1
     batched = total * k / wall_graph
     plain = total * k_plain / wall_plain
-    return {"runs": len(runs), "particles": total, "steps": k, "particle_steps_per_s": batched,
+    return {"runs": len(runs), "particles": total, "steps": k, "neighbours_mean": nb_mean,
+            "particle_steps_per_s": batched,
             "p50_ms": prof["step_ms_p50"], "p99_ms": prof["step_ms_p99"], "tuning": tuning,
             "plain_fp32_one_run_at_a_time": {"particle_steps_per_s": plain, "steps_per_run": k_plain},
             "speedup_vs_plain": batched / plain}

@@ -322,11 +322,14 @@ int reserve(sph_ctx* c, int64_t cap_p, int64_t cap_c, int cap_r, int cap_kb)
     c->cap_kb = cap_kb;
     Carver dry{nullptr, 0, true};
     carve(dry, c, cap_p, c->ncell_cap, cap_kb, cap_r);
-    c->arena_bytes = dry.off + 256;
+    c->arena_bytes = ((dry.off + 255) & ~size_t(255)) + 256;
     HIPTRY(c, hipMalloc(&c->arena, c->arena_bytes));
     Carver real{reinterpret_cast<char*>(c->arena), 0, false};
     carve(real, c, cap_p, c->ncell_cap, cap_kb, cap_r);
-    HIPTRY(c, hipMemset(c->arena, 0, c->arena_bytes));
+    // Cleared on the context's stream: the tables that layout() uploads next go through that
+    // (non-blocking) stream, which is not ordered after work on the null stream.
+    HIPTRY(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->arena), 0u, (int64_t)(c->arena_bytes / 4), c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
     HIPTRY(c, hipMalloc(&c->runs_d, sizeof(RunDev) * cap_r));
     return SPH_OK;
 }
@@ -556,8 +559,9 @@ int sph_pool_set_runs(sph_ctx* c, const int32_t* keep, int32_t n_keep, const sph
     // per-run scalars of the kept runs
     std::vector<double> old_dt(c->nruns), old_time(c->nruns);
     if (c->nruns) {
-        HIPTRY(c, hipMemcpy(old_dt.data(), c->W.dt, sizeof(double) * c->nruns, hipMemcpyDeviceToHost));
-        HIPTRY(c, hipMemcpy(old_time.data(), c->W.time, sizeof(double) * c->nruns, hipMemcpyDeviceToHost));
+        HIPTRY(c, hipMemcpyAsync(old_dt.data(), c->W.dt, sizeof(double) * c->nruns, hipMemcpyDeviceToHost, c->stream));
+        HIPTRY(c, hipMemcpyAsync(old_time.data(), c->W.time, sizeof(double) * c->nruns, hipMemcpyDeviceToHost, c->stream));
+        HIPTRY(c, hipStreamSynchronize(c->stream));
     }
     std::vector<double> dt, time;
     for (int k = 0; k < n_keep; ++k) {

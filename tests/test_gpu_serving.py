@@ -89,3 +89,48 @@ def test_served_results_equal_alone():
         res = srv.results[q.rid]
         _same(a, res.ps.sorted_by_id())
         assert res.conserved["time"] == ca["time"]
+
+
+def test_ensemble_autotuned_graph_matches_alone():
+    """The benchmark's ensemble sequence (prime, autotune, prime, graph capture and replay) on a
+    ragged batch gives each run's physics: neighbour counts as alone, density and positions within
+    the FP32 summation-order tolerance of the run stepped alone with default launch shapes."""
+    sizes = [4096, 9000, 20000, 5200, 13000, 6100, 4500, 16000, 7000, 11000, 4200, 8000]
+    runs = [ic.kelvin_helmholtz(nt, 900 + r) for r, nt in enumerate(sizes)]
+    steps = 4
+    with Context([c for c, _ in runs], [p.n for _, p in runs], precision="mixed") as ctx:
+        for r, (_, p) in enumerate(runs):
+            ctx.upload(r, p)
+        ctx.prime()
+        tuned = ctx.autotune(2)
+        ctx.prime()
+        ctx.graph_capture()
+        ctx.graph_launch(steps)
+        ctx.sync()
+        got = [ctx.download(r).sorted_by_id() for r in range(len(runs))]
+        cons = ctx.conserved()
+    print("tuned:", tuned, flush=True)
+    for (cfg, ps), g, c in zip(runs, got, cons):
+        a, ca = _alone(cfg, ps, steps)
+        assert np.isfinite(c["energy"]) and c["time"] > 0
+        assert abs(c["time"] - ca["time"]) <= 1e-5 * ca["time"]
+        assert abs(np.mean(g.nnb) - np.mean(a.nnb)) <= 0.01 * np.mean(a.nnb)
+        assert np.max(np.abs(g.rho - a.rho) / a.rho) <= 1e-4
+        assert np.max(np.abs(g.pos - a.pos)) <= 1e-6
+
+
+def test_many_runs_keep_their_tables():
+    """A batch of 64 runs in one arena: every run's uploaded state reads back unchanged, and after
+    the first prime every run has its neighbours. (The arena clear at creation must be ordered
+    before the run tables are uploaded on the context's stream.)"""
+    runs = [ic.kelvin_helmholtz(4100 + 37 * r, 1300 + r) for r in range(64)]
+    with Context([c for c, _ in runs], [p.n for _, p in runs], precision="mixed") as ctx:
+        for r, (_, p) in enumerate(runs):
+            ctx.upload(r, p)
+        for r in (0, 1, 31, 63):
+            d = ctx.download(r)
+            assert np.array_equal(d.pos, runs[r][1].pos) and np.array_equal(d.h, runs[r][1].h), r
+        ctx.prime()
+        ctx.sync()
+        means = [ctx.neighbour_total(r) / p.n for r, (_, p) in enumerate(runs)]
+    assert min(means) > 40, means
