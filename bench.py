@@ -65,6 +65,7 @@ def run_main(args):
         ctx.prime()
         ctx.autotune(2)
     ctx.prime()
+    primed = None if args.no_cpu else ctx.download(0).sorted_by_id()  # density + forces of the initial state
     for _ in range(args.warmup):
         ctx.step(1)
     ctx.sync()
@@ -79,7 +80,8 @@ def run_main(args):
     cons = ctx.conserved()[0]
     tuning = ctx.tuning
     ctx.close()
-    return dict(n=n, wall=wall, prof=prof, pairs=pairs, cons=cons, t_ic=t_ic, cfg=cfg, ps=ps, tuning=tuning)
+    return dict(n=n, wall=wall, prof=prof, pairs=pairs, cons=cons, t_ic=t_ic, cfg=cfg, ps=ps, tuning=tuning,
+                primed=primed)
 
 
 def run_ensemble(args):
@@ -131,6 +133,32 @@ def run_ensemble(args):
             "p50_ms": prof["step_ms_p50"], "p99_ms": prof["step_ms_p99"], "tuning": tuning,
             "plain_fp32_one_run_at_a_time": {"particle_steps_per_s": plain, "steps_per_run": k_plain},
             "speedup_vs_plain": batched / plain}
+
+
+def fp64_error(cfg, ps, g):
+    """Error of the GPU's first density and force pass against the FP64 oracle on the same input:
+    density relative, derivatives relative to the sum of |pair terms| (the FP32 summation bound)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ref
+
+    o = oracle_ref.OracleRun(cfg, ps, "f64", scales=True)
+    o.resort()
+    o.density()
+    o.forces()
+    k = o.by_id()
+    t = (o.species[k] == 0) & (o.mask[k] == 1)
+
+    def worst(gv, ov, sc):
+        e = np.abs(np.asarray(gv, np.float64) - ov) / (sc + 1e-300)
+        return float(e[..., t].max())
+
+    out = {"rho": worst(g.rho, o.rho[k], o.rho[k]), "dvdt": worst(g.dvdt, o.dvdt[:, k], o.sc_a[k]),
+           "dudt": worst(g.dudt, o.dudt[k], o.sc_du[k])}
+    if cfg.mhd:
+        out["dbdt"] = worst(g.dbdt, o.dbdt[:, k], o.sc_db[k])
+        out["divb"] = worst(g.divb, o.divb[k], o.sc_divb[k])
+    out["neighbour_counts_equal_fp64"] = bool(np.array_equal(g.nnb, o.nnb[k]))
+    return out
 
 
 def cpu_baseline(cfg, ps, budget_s: float):
@@ -221,6 +249,7 @@ def main():
         out["ensemble"] = run_ensemble(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m["cfg"], m["ps"], args.cpu_seconds)
+        out["fp64_oracle_error"] = fp64_error(m["cfg"], m["ps"], m["primed"])
         out["vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
