@@ -60,6 +60,8 @@ def run_main(args):
     n = ps.n
     t_ic = time.perf_counter() - t0
     ctx = Context([cfg], [n], precision=args.precision)
+    if args.tuning:
+        ctx.set_tuning(**json.loads(args.tuning))
     ctx.upload(0, ps)
     if args.tune:
         ctx.prime()
@@ -189,6 +191,8 @@ def main():
     ap.add_argument("--n", type=int, default=128, help="lattice points per axis (128 -> 2,097,152)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32"])
     ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
+    ap.add_argument("--tuning", default="", help="JSON launch shape for the main workload, e.g. "
+                    "'{\"pair_balance\": 2, \"density_chunk\": 512, \"force_chunk\": 256}'")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-ensemble", action="store_true")

@@ -215,7 +215,7 @@ LaunchCtx lctx(sph_ctx* c)
     L.kmax = std::max(1, std::min(64, c->tune.max_lanes_per_target));
     L.S = c->S[c->cur];
     L.force_chunk = c->tune.force_chunk;
-    L.balance = c->tune.pair_balance ? 1 : 0;
+    L.balance = c->tune.pair_balance < 0 ? 0 : (c->tune.pair_balance > 2 ? 2 : c->tune.pair_balance);
     L.W = c->W;
     L.stream = c->stream;
     return L;
@@ -643,8 +643,14 @@ int sph_set_tuning(sph_ctx* c, const sph_tuning* t)
 {
     if (!c || !t) return SPH_E_INVALID;
     const int dc = t->density_chunk, fc = t->force_chunk;
-    if (dc != 128 && dc != 256 && dc != 512) return fail(c, SPH_E_INVALID, "density_chunk must be 128, 256 or 512");
-    if (fc != 64 && fc != 128 && fc != 256) return fail(c, SPH_E_INVALID, "force_chunk must be 64, 128 or 256");
+    if (t->pair_balance < 0 || t->pair_balance > 2) return fail(c, SPH_E_INVALID, "pair_balance must be 0, 1 or 2");
+    if (t->pair_balance == 2) {
+        if (dc != 512 && dc != 1024) return fail(c, SPH_E_INVALID, "whole-cell density_chunk must be 512 or 1024");
+        if (fc != 128 && fc != 256) return fail(c, SPH_E_INVALID, "whole-cell force_chunk must be 128 or 256");
+    } else {
+        if (dc != 128 && dc != 256 && dc != 512) return fail(c, SPH_E_INVALID, "density_chunk must be 128, 256 or 512");
+        if (fc != 64 && fc != 128 && fc != 256) return fail(c, SPH_E_INVALID, "force_chunk must be 64, 128 or 256");
+    }
     if (t->max_lanes_per_target < 1 || t->max_lanes_per_target > 64)
         return fail(c, SPH_E_INVALID, "max_lanes_per_target must be in [1, 64]");
     c->tune = *t;
@@ -1029,21 +1035,26 @@ int sph_autotune(sph_ctx* c, int32_t n_steps, sph_tuning* best)
     hipEvent_t e0, e1;
     HIPTRY(c, hipEventCreate(&e0));
     HIPTRY(c, hipEventCreate(&e1));
-    const int dcs[2] = {128, 256}, fcs[2] = {64, 128}, ks[2] = {16, 64}, bals[2] = {0, 1};
+    // candidate launch shapes: chunked passes (pair_balance 0/1) and whole-cell passes (2)
+    std::vector<sph_tuning> cands;
+    for (int dc : {128, 256})
+        for (int fc : {64, 128})
+            for (int k : {16, 64})
+                for (int bal : {0, 1}) cands.push_back({dc, fc, t0.waves_per_cu, k, bal});
+    for (int dc : {512, 1024})
+        for (int fc : {128, 256})
+            for (int k : {16, 64}) cands.push_back({dc, fc, t0.waves_per_cu, k, 2});
     const float cfs[2] = {1.0f, 1.15f};
     int rc = SPH_OK;
     for (float cf : cfs) {
         for (int r = 0; r < c->nruns && !rc; ++r) c->cfg[r].cell_factor = std::max(cf, cfg0[r].cell_factor);
-        for (int dc : dcs)
-            for (int fc : fcs)
-                for (int k : ks)
-                    for (int bal : bals) {
+        for (const sph_tuning& cand : cands) {
                     if (rc) break;
                     HIPTRY(c, hipMemcpyAsync(c->arena, backup, half, hipMemcpyDeviceToDevice, c->stream));
                     c->cur = cur0;
                     rc = regrid(c);
                     if (rc) break;
-                    c->tune = {dc, fc, t0.waves_per_cu, k, bal};
+                    c->tune = cand;
                     rc = do_step(c);  // warm-up
                     if (rc) break;
                     HIPTRY(c, hipEventRecord(e0, c->stream));

@@ -466,6 +466,68 @@ __device__ __forceinline__ int compact_group(uint32_t msk, int sb, const LaneMap
     return m.inwave ? gtot : 0;
 }
 
+// K partial sums of a target combined in lane order (deterministic); then one lane per particle
+// writes rho, the neighbour count and the interaction records of the force pass, and the run's
+// fast-speed maximum (cleaning speed) is raised.
+template <bool MHD>
+__device__ __forceinline__ void density_finish(const RunDev& R, int r, StateBuf& S, WorkBuf& W, const LaneMap& lm,
+                                               float acc, uint32_t cnt, bool valid, bool tgt, uint8_t spc,
+                                               uint8_t live, int i, float m_own, float h, float invh, float uu,
+                                               float vx, float vy, float vz, float rho_in, float bx, float by,
+                                               float bz, float psi, float gam, int kern)
+{
+    const int K = lm.K, k = lm.k;
+    const int lane = threadIdx.x;
+    // combine the K partial sums in lane order (deterministic)
+    float acc_t = 0.0f;
+    uint32_t cnt_t = 0;
+    for (int kk = 0; kk < K; ++kk) {
+        const int src = lm.gfirst + kk < 64 ? lm.gfirst + kk : 63;
+        acc_t += __shfl(acc, src, 64);
+        cnt_t += __shfl(cnt, src, 64);
+    }
+    // epilogue: one lane per particle writes rho and its interaction records
+    float cmax = 0.0f;
+    if (k == 0 && valid && is_source(spc, live)) {
+#pragma clang fp contract(off)
+        const float sig = R.sigma;
+        const float wfac = sig / (h * h * h);
+        const float gfac = (kern == 2) ? -5.0f * sig / (h * h * h * h * h) : sig / (h * h * h * h);
+        float rho;
+        if (tgt) {
+            rho = wfac * (acc_t + m_own * R.f0);
+            S.rho[i] = rho;
+            W.nnb[i] = cnt_t;
+        } else {
+            rho = rho_in;
+        }
+        const float P = (gam - 1.0f) * rho * uu;
+        const float invrho = 1.0f / rho;
+        const float B2 = bx * bx + by * by + bz * bz;
+        const float bdr = invrho * invrho;
+        const float ph = P * bdr;
+        const float pr = (P + 0.5f * B2) * bdr;
+        const float psr = psi * bdr;
+        const float c = sqrtf((gam * P + B2) * invrho);
+        W.P1[i] = make_float4(vx, vy, vz, c);
+        W.P2[i] = make_float4(h, invh, gfac, rho);
+        W.P3[i] = make_float4(pr, ph, bdr, psr);
+        if (MHD) W.P4[i] = make_float4(bx, by, bz, psi);
+        if (tgt) cmax = c;
+    }
+    if (k == 0 && valid && !tgt) W.nnb[i] = 0;
+    if (MHD && R.mhd && R.cleaning) {
+        for (int off = 32; off >= 1; off >>= 1) cmax = fmaxf(cmax, __shfl_xor(cmax, off, 64));
+        // every wave of the run targets this one word: test before the atomic, so that only a
+        // wave holding a new maximum pays for one (the value only grows, so a stale read costs
+        // at most a redundant atomic)
+        if (lane == 0 && cmax > 0.0f) {
+            const uint32_t bits = __float_as_uint(cmax);
+            if (bits > __atomic_load_n(&W.chbits[r], __ATOMIC_RELAXED)) atomicMax(&W.chbits[r], bits);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // density pass: cull of the 27-supercell candidates staged in LDS, density sum, candidate masks,
 // and the per-particle interaction records of the force pass
@@ -580,54 +642,8 @@ __global__ __launch_bounds__(64) void k_density(const RunDev* __restrict__ runs,
             __syncthreads();
         }
         if (keep) mw.flush();
-        // combine the K partial sums in lane order (deterministic)
-        float acc_t = 0.0f;
-        uint32_t cnt_t = 0;
-        for (int kk = 0; kk < K; ++kk) {
-            const int src = lm.gfirst + kk < 64 ? lm.gfirst + kk : 63;
-            acc_t += __shfl(acc, src, 64);
-            cnt_t += __shfl(cnt, src, 64);
-        }
-        // epilogue: one lane per particle writes rho and its interaction records
-        float cmax = 0.0f;
-        if (k == 0 && valid && is_source(spc, live)) {
-#pragma clang fp contract(off)
-            const float sig = R.sigma;
-            const float wfac = sig / (h * h * h);
-            const float gfac = (kern == 2) ? -5.0f * sig / (h * h * h * h * h) : sig / (h * h * h * h);
-            float rho;
-            if (tgt) {
-                rho = wfac * (acc_t + own.w * R.f0);
-                S.rho[i] = rho;
-                W.nnb[i] = cnt_t;
-            } else {
-                rho = rho_in;
-            }
-            const float P = (gam - 1.0f) * rho * uu;
-            const float invrho = 1.0f / rho;
-            const float B2 = bx * bx + by * by + bz * bz;
-            const float bdr = invrho * invrho;
-            const float ph = P * bdr;
-            const float pr = (P + 0.5f * B2) * bdr;
-            const float psr = psi * bdr;
-            const float c = sqrtf((gam * P + B2) * invrho);
-            W.P1[i] = make_float4(vx, vy, vz, c);
-            W.P2[i] = make_float4(h, invh, gfac, rho);
-            W.P3[i] = make_float4(pr, ph, bdr, psr);
-            if (MHD) W.P4[i] = make_float4(bx, by, bz, psi);
-            if (tgt) cmax = c;
-        }
-        if (k == 0 && valid && !tgt) W.nnb[i] = 0;
-        if (MHD && R.mhd && R.cleaning) {
-            for (int off = 32; off >= 1; off >>= 1) cmax = fmaxf(cmax, __shfl_xor(cmax, off, 64));
-            // every wave of the run targets this one word: test before the atomic, so that only a
-            // wave holding a new maximum pays for one (the value only grows, so a stale read costs
-            // at most a redundant atomic)
-            if (lane == 0 && cmax > 0.0f) {
-                const uint32_t bits = __float_as_uint(cmax);
-                if (bits > __atomic_load_n(&W.chbits[r], __ATOMIC_RELAXED)) atomicMax(&W.chbits[r], bits);
-            }
-        }
+        density_finish<MHD>(R, r, S, W, lm, acc, cnt, valid, tgt, spc, live, i, own.w, h, invh, uu, vx, vy, vz, rho_in,
+                            bx, by, bz, psi, gam, kern);
     }
 }
 
@@ -695,6 +711,83 @@ __device__ __forceinline__ void force_pair(const float4& o0, const float4& o1, c
             A.PSx += fps * dx;
             A.PSy += fps * dy;
             A.PSz += fps * dz;
+        }
+    }
+}
+
+// K partial force sums of a target combined in lane order (deterministic), field by field; then
+// one lane per target writes its derivatives, signal speed and div B.
+template <bool MHD>
+__device__ __forceinline__ void force_finish(const RunDev& R, WorkBuf& W, const LaneMap& lm, ForceAcc& A, bool tgt,
+                                             int i, const float4& o1, const float4& o2, const float4& o3,
+                                             const float4& o4, bool mhd, bool clean, float borve, float ch)
+{
+    const int K = lm.K, k = lm.k;
+    // combine the K partial sums in lane order (deterministic), field by field in place
+    auto red = [&](float v) {
+        float s = 0.0f;
+        for (int kk = 0; kk < K; ++kk) s += __shfl(v, lm.gfirst + kk < 64 ? lm.gfirst + kk : 63, 64);
+        return s;
+    };
+    ForceAcc& T = A;
+    T.ax = red(A.ax);
+    T.ay = red(A.ay);
+    T.az = red(A.az);
+    T.DV = red(A.DV);
+    T.AVU = red(A.AVU);
+    {
+        float vm = 0.0f;
+        for (int kk = 0; kk < K; ++kk) vm = fmaxf(vm, __shfl(A.vmax, lm.gfirst + kk < 64 ? lm.gfirst + kk : 63, 64));
+        T.vmax = vm;
+    }
+    if (MHD) {
+        T.Ti = red(A.Ti);
+        T.Tj = red(A.Tj);
+        T.TBx = red(A.TBx);
+        T.TBy = red(A.TBy);
+        T.TBz = red(A.TBz);
+        T.VBx = red(A.VBx);
+        T.VBy = red(A.VBy);
+        T.VBz = red(A.VBz);
+        T.DB = red(A.DB);
+        T.PSx = red(A.PSx);
+        T.PSy = red(A.PSy);
+        T.PSz = red(A.PSz);
+    }
+    if (k == 0 && tgt) {
+        const float rho = o2.w;
+        const float invrho = 1.0f / rho;
+        float ax = T.ax, ay = T.ay, az = T.az;
+        if (MHD && mhd) {
+            const float tcoef = T.Ti - borve * (T.Ti + T.Tj);
+            ax = ax + T.TBx + o4.x * tcoef;
+            ay = ay + T.TBy + o4.y * tcoef;
+            az = az + T.TBz + o4.z * tcoef;
+        }
+        W.dvdt[0][i] = ax;
+        W.dvdt[1][i] = ay;
+        W.dvdt[2][i] = az;
+        W.dudt[i] = o3.y * T.DV + 0.5f * T.AVU;
+        W.vsig[i] = fmaxf(2.0f * o1.w, T.vmax);
+        if (MHD && mhd) {
+            const float divB = -T.DB * invrho;
+            const float divv = -T.DV * invrho;
+            float dbx = -(T.VBx - o4.x * T.DV) * invrho;
+            float dby = -(T.VBy - o4.y * T.DV) * invrho;
+            float dbz = -(T.VBz - o4.z * T.DV) * invrho;
+            float dpsi = 0.0f;
+            if (clean) {
+                dbx -= rho * T.PSx;
+                dby -= rho * T.PSy;
+                dbz -= rho * T.PSz;
+                const float psi = o4.w;
+                dpsi = -ch * ch * divB - psi * R.clean_sigma * ch * o2.y - 0.5f * psi * divv;
+            }
+            W.dbdt[0][i] = dbx;
+            W.dbdt[1][i] = dby;
+            W.dbdt[2][i] = dbz;
+            W.dpsidt[i] = dpsi;
+            W.divb[i] = divB;
         }
     }
 }
@@ -804,73 +897,264 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
             }
             __syncthreads();
         }
-        // combine the K partial sums in lane order (deterministic), field by field in place
-        auto red = [&](float v) {
-            float s = 0.0f;
-            for (int kk = 0; kk < K; ++kk) s += __shfl(v, lm.gfirst + kk < 64 ? lm.gfirst + kk : 63, 64);
-            return s;
-        };
-        ForceAcc& T = A;
-        T.ax = red(A.ax);
-        T.ay = red(A.ay);
-        T.az = red(A.az);
-        T.DV = red(A.DV);
-        T.AVU = red(A.AVU);
-        {
-            float vm = 0.0f;
-            for (int kk = 0; kk < K; ++kk) vm = fmaxf(vm, __shfl(A.vmax, lm.gfirst + kk < 64 ? lm.gfirst + kk : 63, 64));
-            T.vmax = vm;
+        force_finish<MHD>(R, W, lm, A, tgt, i, o1, o2, o3, o4, mhd, clean, borve, ch);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Whole-cell pair passes (tuning pair_balance = 2).
+//
+// Lane (target t, sub-lane k) owns the contiguous candidate range [k*Lr, min((k+1)*Lr, total)) of
+// the 27-supercell virtual list, Lr = ceil(total / K). The density pass stages the positions of
+// all candidates at once (chunks of CAPD only for very large stencils), culls its range and stores
+// the accepted bits of the range (bit b <-> candidate k*Lr + b). Because the ranges of a target's
+// K lanes are consecutive, concatenating their accepted candidates gives each target's neighbour
+// list sorted by candidate index. The force pass builds these lists once per supercell in LDS, then
+// stages the records chunk by chunk: within a chunk a target's pairs are one contiguous run of its
+// list, dealt round-robin to its K lanes, with no per-chunk compaction. Cells whose lists do not
+// fit (or whose masks were not stored) take a direct per-lane loop over their own bits.
+// ------------------------------------------------------------------------------------------------
+constexpr int kCellListCap = 2048;  // uint16 entries of the per-supercell neighbour lists
+
+// Popcount of bits [lo, hi) of a lane's mask row (word w at row[w * 64]).
+__device__ __forceinline__ int mask_count(const uint32_t* row, int lo, int hi)
+{
+    int n = 0;
+    for (int b = lo; b < hi;) {
+        const int w = b >> 5, off = b & 31;
+        const int take = (32 - off) < (hi - b) ? (32 - off) : (hi - b);
+        uint32_t m = row[w * 64] >> off;
+        if (take < 32) m &= (1u << take) - 1u;
+        n += __popc(m);
+        b += take;
+    }
+    return n;
+}
+
+// Bits [lo, lo + lim) of a lane's mask row, LSB first (lim <= 32).
+__device__ __forceinline__ uint32_t mask_bits(const uint32_t* row, int lo, int lim)
+{
+    const int w = lo >> 5, off = lo & 31;
+    uint32_t a = row[w * 64] >> off;
+    if (off && off + lim > 32) a |= row[(w + 1) * 64] << (32 - off);
+    return lim == 32 ? a : (a & ((1u << lim) - 1u));
+}
+
+template <int CAPD, bool MIXED, bool MHD>
+__global__ __launch_bounds__(64) void k_density_cell(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
+                                                     int nblk_total, StateBuf S, WorkBuf W, int kmax)
+{
+    __shared__ NbList L;
+    __shared__ float4 sp[CAPD];
+    const int cell = xcd_cell(blockIdx.x, nblk_total, ncell_tot);
+    if (cell < 0) return;
+    const int t0 = (int)W.cs[cell];
+    const int nt = (int)W.cs[cell + 1] - t0;
+    if (nt == 0) return;
+    const int r = find_run_by_cell(runs, nruns, cell);
+    const RunDev& R = runs[r];
+    build_nblist<MIXED>(R, cell - R.c0, W.cs, L);
+    const int total = L.off[27];
+    const int lane = threadIdx.x;
+    const LaneMap lm = lane_map(nt, kmax);
+    const int K = lm.K, k = lm.k;
+    const int Lr = (total + K - 1) / K;
+    const bool keep = nt <= lm.tpg && Lr <= 32 * kMaskWords;
+    const int kern = R.kernel;
+    const float H2 = R.H2;
+    const float gam = R.gamma;
+    MaskWriter mw;
+    mw.row = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
+    mw.pend = 0;
+    mw.pend_w = -1;
+
+    for (int tg = 0; tg < nt; tg += lm.tpg) {
+        const int t = tg + lm.tl;
+        const bool valid = lm.inwave && t < nt;
+        const int i = t0 + (valid ? t : 0);
+        const uint8_t live = S.mask[i], spc = S.species[i];
+        const bool tgt = valid && is_target(spc, live);
+        const float4 own = W.P0[i];
+        const float h = S.h[i];
+        const float uu = S.u[i];
+        const float vx = S.vel[0][i], vy = S.vel[1][i], vz = S.vel[2][i];
+        const float rho_in = S.rho[i];
+        float bx = 0.f, by = 0.f, bz = 0.f, psi = 0.f;
+        if (MHD && R.mhd) {
+            bx = S.B[0][i];
+            by = S.B[1][i];
+            bz = S.B[2][i];
+            psi = S.psi[i];
         }
-        if (MHD) {
-            T.Ti = red(A.Ti);
-            T.Tj = red(A.Tj);
-            T.TBx = red(A.TBx);
-            T.TBy = red(A.TBy);
-            T.TBz = red(A.TBz);
-            T.VBx = red(A.VBx);
-            T.VBy = red(A.VBy);
-            T.VBz = red(A.VBz);
-            T.DB = red(A.DB);
-            T.PSx = red(A.PSx);
-            T.PSy = red(A.PSy);
-            T.PSz = red(A.PSz);
-        }
-        if (k == 0 && tgt) {
-            const float rho = o2.w;
-            const float invrho = 1.0f / rho;
-            float ax = T.ax, ay = T.ay, az = T.az;
-            if (MHD && mhd) {
-                const float tcoef = T.Ti - borve * (T.Ti + T.Tj);
-                ax = ax + T.TBx + o4.x * tcoef;
-                ay = ay + T.TBy + o4.y * tcoef;
-                az = az + T.TBz + o4.z * tcoef;
+        const float invh = 1.0f / h;
+        const int self_v = L.off[13] + t;
+        const int v0 = lm.inwave ? k * Lr : total;
+        const int v1 = v0 + Lr < total ? v0 + Lr : total;
+        float acc = 0.0f;
+        uint32_t cnt = 0;
+        for (int base = 0; base < total; base += CAPD) {
+            const int top = base + CAPD < total ? base + CAPD : total;
+            for (int v = base + lane; v < top; v += 64) {
+                const int sg = nb_segment(L, v);
+                float4 p = W.P0[L.start[sg] + (v - L.off[sg])];
+                p.x += L.sh[sg][0];
+                p.y += L.sh[sg][1];
+                p.z += L.sh[sg][2];
+                sp[v - base] = p;
             }
-            W.dvdt[0][i] = ax;
-            W.dvdt[1][i] = ay;
-            W.dvdt[2][i] = az;
-            W.dudt[i] = o3.y * T.DV + 0.5f * T.AVU;
-            W.vsig[i] = fmaxf(2.0f * o1.w, T.vmax);
-            if (MHD && mhd) {
-                const float divB = -T.DB * invrho;
-                const float divv = -T.DV * invrho;
-                float dbx = -(T.VBx - o4.x * T.DV) * invrho;
-                float dby = -(T.VBy - o4.y * T.DV) * invrho;
-                float dbz = -(T.VBz - o4.z * T.DV) * invrho;
-                float dpsi = 0.0f;
-                if (clean) {
-                    dbx -= rho * T.PSx;
-                    dby -= rho * T.PSy;
-                    dbz -= rho * T.PSz;
-                    const float psi = o4.w;
-                    dpsi = -ch * ch * divB - psi * R.clean_sigma * ch * o2.y - 0.5f * psi * divv;
+            __syncthreads();
+            const int a = v0 > base ? v0 : base;
+            const int b = v1 < top ? v1 : top;
+            for (int cc = a; cc < b; cc += 32) {
+                const int lim = b - cc < 32 ? b - cc : 32;
+                uint32_t msk = cull32(sp, cc - base, 1, lim, own.x, own.y, own.z, H2);
+                // the self pair passes the cull; its density term is the self term added at the end
+                if (self_v >= cc && self_v < cc + lim) msk &= ~(1u << (self_v - cc));
+                if (!tgt) msk = 0;
+                if (keep) mw.put(msk, cc - v0, lim);
+                cnt += (uint32_t)__popc(msk);
+                while (msk) {
+                    const int ii = __builtin_ctz(msk);
+                    msk &= msk - 1;
+                    const float4 p = sp[cc - base + ii];
+                    float dx, dy, dz;
+                    const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
+                    const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
+                    acc += p.w * kshape(kern, r2 * invR * invh);
                 }
-                W.dbdt[0][i] = dbx;
-                W.dbdt[1][i] = dby;
-                W.dbdt[2][i] = dbz;
-                W.dpsidt[i] = dpsi;
-                W.divb[i] = divB;
             }
+            __syncthreads();
         }
+        if (keep) mw.flush();
+        density_finish<MHD>(R, r, S, W, lm, acc, cnt, valid, tgt, spc, live, i, own.w, h, invh, uu, vx, vy, vz,
+                            rho_in, bx, by, bz, psi, gam, kern);
+    }
+}
+
+template <int CH, bool MIXED, bool MHD>
+__global__ __launch_bounds__(64) void k_forces_cell(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
+                                                    int nblk_total, StateBuf S, WorkBuf W, int kmax)
+{
+    constexpr int NP = MHD ? 5 : 4;
+    __shared__ NbList L;
+    __shared__ float4 sp[NP][CH];
+    __shared__ uint16_t lst[kCellListCap];
+    const int cell = xcd_cell(blockIdx.x, nblk_total, ncell_tot);
+    if (cell < 0) return;
+    const int t0 = (int)W.cs[cell];
+    const int nt = (int)W.cs[cell + 1] - t0;
+    if (nt == 0) return;
+    const int r = find_run_by_cell(runs, nruns, cell);
+    const RunDev& R = runs[r];
+    build_nblist<MIXED>(R, cell - R.c0, W.cs, L);
+    const int total = L.off[27];
+    const int lane = threadIdx.x;
+    const LaneMap lm = lane_map(nt, kmax);
+    const int K = lm.K, k = lm.k;
+    const int Lr = (total + K - 1) / K;
+    const bool keep = nt <= lm.tpg && Lr <= 32 * kMaskWords;  // the density pass stored masks
+    const int kern = R.kernel;
+    const float H2 = R.H2;
+    const float alpha = R.alpha, beta = R.beta;
+    const bool mhd = MHD && R.mhd;
+    const bool clean = mhd && R.cleaning;
+    const float borve = mhd ? R.borve : 0.0f;
+    const float ch = __uint_as_float(W.chbits[r]);
+    const uint32_t* mrow = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
+
+    for (int tg = 0; tg < nt; tg += lm.tpg) {
+        const int t = tg + lm.tl;
+        const bool valid = lm.inwave && t < nt;
+        const int i = t0 + (valid ? t : 0);
+        const bool tgt = valid && is_target(S.species[i], S.mask[i]);
+        const float4 o0 = W.P0[i], o1 = W.P1[i], o2 = W.P2[i], o3 = W.P3[i];
+        float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (MHD) o4 = W.P4[i];
+        const int self_v = L.off[13] + t;
+        const int v0 = lm.inwave ? k * Lr : total;
+        const int v1 = v0 + Lr < total ? v0 + Lr : total;
+        ForceAcc A = {};
+
+        // neighbour lists from the stored masks: lane order = candidate order within a target
+        const int mine = (keep && tgt) ? mask_count(mrow, 0, v1 - v0) : 0;
+        const int incl = wave_incl_scan(mine);
+        const bool listed = keep && __shfl(incl, 63, 64) <= kCellListCap;
+        int lo = 0;
+        if (listed) {
+            int pos = incl - mine;
+            for (int w = 0; w * 32 < v1 - v0 && mine > 0; ++w) {
+                const int lim = (v1 - v0) - w * 32 < 32 ? (v1 - v0) - w * 32 : 32;
+                uint32_t m = mask_bits(mrow, w * 32, lim);
+                while (m) {
+                    const int ii = __builtin_ctz(m);
+                    m &= m - 1;
+                    lst[pos++] = (uint16_t)(v0 + w * 32 + ii);
+                }
+            }
+            const int gbefore = lm.gfirst > 0 ? __shfl(incl, lm.gfirst - 1, 64) : 0;
+            lo = gbefore;  // first entry of this target's list
+        }
+
+        for (int base = 0; base < total; base += CH) {
+            const int top = base + CH < total ? base + CH : total;
+            for (int v = base + lane; v < top; v += 64) {
+                const int e = v - base;
+                const int sg = nb_segment(L, v);
+                const int j = L.start[sg] + (v - L.off[sg]);
+                float4 p = W.P0[j];
+                p.x += L.sh[sg][0];
+                p.y += L.sh[sg][1];
+                p.z += L.sh[sg][2];
+                sp[0][e] = p;
+                sp[1][e] = W.P1[j];
+                sp[2][e] = W.P2[j];
+                sp[3][e] = W.P3[j];
+                if (MHD) sp[4][e] = W.P4[j];
+            }
+            __syncthreads();
+            if (listed) {
+                // this target's entries below `top`: counted from each sub-lane's own bits
+                const int a = v0 > base ? v0 : base;
+                const int b = v1 < top ? v1 : top;
+                const int in_chunk = (tgt && b > a) ? mask_count(mrow, a - v0, b - v0) : 0;
+                const int ci = wave_incl_scan(in_chunk);
+                const int cb = lm.gfirst > 0 ? __shfl(ci, lm.gfirst - 1, 64) : 0;
+                const int g = lm.inwave ? __shfl(ci, lm.glast, 64) - cb : 0;
+                for (int q = k; q < g; q += K) {
+                    const int e = (int)lst[lo + q] - base;
+                    float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (MHD) q4 = sp[4][e];
+                    force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
+                                    beta, mhd, clean, A);
+                }
+                lo += g;
+            } else {
+                const int a = v0 > base ? v0 : base;
+                const int b = v1 < top ? v1 : top;
+                for (int cc = a; cc < b; cc += 32) {
+                    const int lim = b - cc < 32 ? b - cc : 32;
+                    uint32_t msk;
+                    if (keep) {
+                        msk = mask_bits(mrow, cc - v0, lim);
+                    } else {
+                        msk = cull32(sp[0], cc - base, 1, lim, o0.x, o0.y, o0.z, H2);
+                        if (self_v >= cc && self_v < cc + lim) msk &= ~(1u << (self_v - cc));
+                    }
+                    if (!tgt) msk = 0;
+                    while (msk) {
+                        const int ii = __builtin_ctz(msk);
+                        msk &= msk - 1;
+                        const int e = cc - base + ii;
+                        float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (MHD) q4 = sp[4][e];
+                        force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
+                                        beta, mhd, clean, A);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        force_finish<MHD>(R, W, lm, A, tgt, i, o1, o2, o3, o4, mhd, clean, borve, ch);
     }
 }
 
@@ -1077,9 +1361,46 @@ static void density_ch(const LaunchCtx& c, int nblk)
     }
 }
 
+template <int CAPD>
+static void density_cell_ch(const LaunchCtx& c, int nblk)
+{
+    if (c.mixed) {
+        if (c.has_mhd)
+            k_density_cell<CAPD, true, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
+        else
+            k_density_cell<CAPD, true, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
+    } else {
+        if (c.has_mhd)
+            k_density_cell<CAPD, false, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
+        else
+            k_density_cell<CAPD, false, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
+    }
+}
+
+template <int CH>
+static void forces_cell_ch(const LaunchCtx& c, int nblk)
+{
+    if (c.mixed) {
+        if (c.has_mhd)
+            k_forces_cell<CH, true, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
+        else
+            k_forces_cell<CH, true, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
+    } else {
+        if (c.has_mhd)
+            k_forces_cell<CH, false, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
+        else
+            k_forces_cell<CH, false, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
+    }
+}
+
 hipError_t launch_density(const LaunchCtx& c)
 {
     const int nblk = nblk_xcd(c.ncell_tot);
+    if (c.balance == 2) {
+        if (c.density_chunk >= 1024) density_cell_ch<1024>(c, nblk);
+        else density_cell_ch<512>(c, nblk);
+        return hipGetLastError();
+    }
     switch (c.density_chunk) {
     case 128: density_ch<128>(c, nblk); break;
     case 512: density_ch<512>(c, nblk); break;
@@ -1107,6 +1428,11 @@ static void forces_ch(const LaunchCtx& c, int nblk)
 hipError_t launch_forces(const LaunchCtx& c)
 {
     const int nblk = nblk_xcd(c.ncell_tot);
+    if (c.balance == 2) {
+        if (c.force_chunk >= 256) forces_cell_ch<256>(c, nblk);
+        else forces_cell_ch<128>(c, nblk);
+        return hipGetLastError();
+    }
     switch (c.force_chunk) {
     case 64: forces_ch<64>(c, nblk); break;
     case 256: forces_ch<256>(c, nblk); break;

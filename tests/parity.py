@@ -30,8 +30,11 @@ def assert_scaled(g, o, scale, what, rtol=RTOL, mask=None):
     return worst
 
 
-def gpu_stages(cfg, ps: ParticleSet, precision: str, stages=("resort", "density", "forces")) -> ParticleSet:
+def gpu_stages(cfg, ps: ParticleSet, precision: str, stages=("resort", "density", "forces"),
+               tuning: dict = None) -> ParticleSet:
     with Context([cfg], [ps.n], precision=precision) as ctx:
+        if tuning:
+            ctx.set_tuning(**tuning)
         ctx.upload(0, ps)
         for s in stages:
             getattr(ctx, s)()

@@ -216,3 +216,65 @@ def test_fp64_oracle_error_mixed():
     assert float(np.max(rel_rho)) < 1e-5
     e = np.abs(g.dvdt - o.dvdt[:, order]) / o.sc_a[order]
     assert float(np.max(e)) < 1e-5
+
+
+# Launch shapes of the pair passes: chunked with per-target lane groups (pair_balance 0), with
+# compacted round-robin pairs (1), and whole-cell lists (2). All must give the oracle's results.
+MODES = {
+    "chunked": dict(density_chunk=128, force_chunk=64, max_lanes_per_target=64, pair_balance=0),
+    "balanced": dict(density_chunk=256, force_chunk=128, max_lanes_per_target=16, pair_balance=1),
+    "cell512": dict(density_chunk=512, force_chunk=128, max_lanes_per_target=64, pair_balance=2),
+    "cell1024": dict(density_chunk=1024, force_chunk=256, max_lanes_per_target=16, pair_balance=2),
+}
+
+
+def _clump_case():
+    """A sparse periodic lattice (about 2 particles per supercell) with a tight clump of 50
+    particles inside one supercell: that cell has at most 64 targets and its masks are stored, but
+    its accepted pairs (> 2450) exceed the whole-cell list, so it takes the direct per-lane path."""
+    n, nclump = 10, 50
+    dx = 1.0 / n
+    rng = np.random.default_rng(41)
+    lat = ic.lattice(n, n, n, (0, 0, 0), dx, rng)
+    centre = np.array([[0.52], [0.47], [0.55]])
+    pos = np.concatenate([lat, centre + rng.uniform(-0.01, 0.01, size=(3, nclump))], axis=1)
+    ps = ParticleSet(pos.shape[1])
+    ps.id = np.arange(ps.n, dtype=np.uint64)
+    ps.pos = pos
+    h0 = 0.06
+    ps.mass = np.full(ps.n, dx ** 3, dtype=np.float32)
+    ps.h = np.full(ps.n, h0, dtype=np.float32)
+    ps.u = np.full(ps.n, 1.5, dtype=np.float32)
+    ps.rho = np.ones(ps.n, dtype=np.float32)
+    ps.vel = rng.normal(0, 0.1, size=(3, ps.n)).astype(np.float32)
+    ps.B = np.stack([np.full(ps.n, 0.1), rng.normal(0, 0.02, ps.n), np.zeros(ps.n)]).astype(np.float32)
+    cfg = capi.make_config(periodic=(1, 1, 1), kernel=capi.KERNEL_WENDLAND, h0=h0, mhd=1, cleaning=1, borve=1)
+    return cfg, ps
+
+
+def _big_cells_case():
+    """Supercells of edge >= 2H on a 16^3 lattice: ~150 targets (several target groups) and ~4000
+    candidates per cell (several chunks of the staging buffer; masks not kept, so the force pass
+    culls again)."""
+    cfg, ps = ic.mhd_turbulence(n=16, seed=43)
+    cfg.cell_factor = 2.0
+    return cfg, ps
+
+
+CASES = {
+    "turbulence": lambda: _turb(16, 17),
+    "sod_walls": lambda: ic.sod_tube(seed=5, nx_left=32, cross=4),
+    "clump": _clump_case,
+    "big_cells": _big_cells_case,
+}
+
+
+@pytest.mark.parametrize("mode", sorted(MODES))
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("prec", PRECS)
+def test_forces_parity_every_launch_shape(mode, case, prec):
+    cfg, ps = CASES[case]()
+    print(f"shape {mode} case {case} prec {prec} n={ps.n}", flush=True)
+    g = gpu_stages(cfg, ps, GPREC[prec], tuning=MODES[mode])
+    o = oracle_stages(cfg, ps, prec)
+    compare_forces(g, o, bool(cfg.mhd))
