@@ -474,7 +474,7 @@ int FN(orc_forces)(const sph_run_config* cfg, int32_t prec, orc_state* st, const
                     ax += cax;
                     ay += cay;
                     az += caz;
-                    double cnorm = sqrt((double)cax * cax + (double)cay * cay + (double)caz * caz);
+                    double cnorm = sqrt((double)cax * (double)cax + (double)cay * (double)cay + (double)caz * (double)caz);
                     if (mhd) {
                         REAL Bir = bi[0] * d[0] + bi[1] * d[1] + bi[2] * d[2];
                         REAL Bjr = Bx[j] * d[0] + By[j] * d[1] + Bz[j] * d[2];
@@ -491,11 +491,11 @@ int FN(orc_forces)(const sph_run_config* cfg, int32_t prec, orc_state* st, const
                         VBz += wv * dvz;
                         REAL db = ms_i * (Bir - Bjr);
                         DB += db;
-                        double bn_i = sqrt((double)bi[0] * bi[0] + (double)bi[1] * bi[1] + (double)bi[2] * bi[2]);
-                        double bn_j = sqrt((double)Bx[j] * Bx[j] + (double)By[j] * By[j] + (double)Bz[j] * Bz[j]);
+                        double bn_i = sqrt((double)bi[0] * (double)bi[0] + (double)bi[1] * (double)bi[1] + (double)bi[2] * (double)bi[2]);
+                        double bn_j = sqrt((double)Bx[j] * (double)Bx[j] + (double)By[j] * (double)By[j] + (double)Bz[j] * (double)Bz[j]);
                         cnorm += fabs((double)ti) * bn_i + fabs((double)tj) * bn_j
                                  + (double)borve * fabs((double)ti + (double)tj) * bn_i;
-                        double dvn = sqrt((double)dvx * dvx + (double)dvy * dvy + (double)dvz * dvz);
+                        double dvn = sqrt((double)dvx * (double)dvx + (double)dvy * (double)dvy + (double)dvz * (double)dvz);
                         sdb += (fabs((double)wv) * dvn + bn_i * fabs((double)(ms_i * vr))) / (double)rho[i];
                         sdiv += fabs((double)db) / (double)rho[i];
                         if (clean) {
@@ -503,7 +503,7 @@ int FN(orc_forces)(const sph_run_config* cfg, int32_t prec, orc_state* st, const
                             PSx += fps * d[0];
                             PSy += fps * d[1];
                             PSz += fps * d[2];
-                            double dn = sqrt((double)d[0] * d[0] + (double)d[1] * d[1] + (double)d[2] * d[2]);
+                            double dn = sqrt((double)d[0] * (double)d[0] + (double)d[1] * (double)d[1] + (double)d[2] * (double)d[2]);
                             sdb += (double)rho[i] * fabs((double)fps) * dn;
                         }
                     }
