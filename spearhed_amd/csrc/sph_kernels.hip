@@ -1091,8 +1091,10 @@ __global__ __launch_bounds__(64) void k_forces_cell(const RunDev* __restrict__ r
                     lst[pos++] = (uint16_t)(v0 + w * 32 + ii);
                 }
             }
-            const int gbefore = lm.gfirst > 0 ? __shfl(incl, lm.gfirst - 1, 64) : 0;
-            lo = gbefore;  // first entry of this target's list
+            // every lane takes part in the shuffle (a lane reading from an inactive lane does not
+            // get that lane's value): clamp the source, then select
+            const int pre = __shfl(incl, lm.gfirst > 0 ? lm.gfirst - 1 : 0, 64);
+            lo = lm.gfirst > 0 ? pre : 0;  // first entry of this target's list
         }
 
         for (int base = 0; base < total; base += CH) {
@@ -1118,8 +1120,9 @@ __global__ __launch_bounds__(64) void k_forces_cell(const RunDev* __restrict__ r
                 const int b = v1 < top ? v1 : top;
                 const int in_chunk = (tgt && b > a) ? mask_count(mrow, a - v0, b - v0) : 0;
                 const int ci = wave_incl_scan(in_chunk);
-                const int cb = lm.gfirst > 0 ? __shfl(ci, lm.gfirst - 1, 64) : 0;
-                const int g = lm.inwave ? __shfl(ci, lm.glast, 64) - cb : 0;
+                const int cpre = __shfl(ci, lm.gfirst > 0 ? lm.gfirst - 1 : 0, 64);
+                const int clast = __shfl(ci, lm.glast, 64);
+                const int g = lm.inwave ? clast - (lm.gfirst > 0 ? cpre : 0) : 0;
                 for (int q = k; q < g; q += K) {
                     const int e = (int)lst[lo + q] - base;
                     float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
