@@ -39,7 +39,7 @@ struct sph_ctx {
     WorkBuf W = {};
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    sph_tuning tune = {128, 64, 0, 64, 1};  // autotuned on the 2M-particle MHD turbulence step
+    sph_tuning tune = {128, 64, 0, 64, 0};  // fastest measured on the 2M-particle MHD turbulence step
     void* arena = nullptr;
     size_t arena_bytes = 0;
     bool binned = false;   // keys + histogram valid for the current positions
@@ -216,6 +216,9 @@ LaunchCtx lctx(sph_ctx* c)
     L.S = c->S[c->cur];
     L.force_chunk = c->tune.force_chunk;
     L.balance = c->tune.pair_balance < 0 ? 0 : (c->tune.pair_balance > 2 ? 2 : c->tune.pair_balance);
+    L.kern = c->nruns > 0 ? c->cfg[0].kernel : -1;
+    for (int r = 1; r < c->nruns; ++r)
+        if (c->cfg[r].kernel != L.kern) L.kern = -1;
     L.W = c->W;
     L.stream = c->stream;
     return L;

@@ -532,7 +532,7 @@ __device__ __forceinline__ void density_finish(const RunDev& R, int r, StateBuf&
 // density pass: cull of the 27-supercell candidates staged in LDS, density sum, candidate masks,
 // and the per-particle interaction records of the force pass
 // ------------------------------------------------------------------------------------------------
-template <int CH, bool MIXED, bool MHD>
+template <int CH, bool MIXED, bool MHD, int KT>
 __global__ __launch_bounds__(64) void k_density(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
                                                 int nblk_total, StateBuf S, WorkBuf W, int kmax, int balance)
 {
@@ -555,7 +555,7 @@ __global__ __launch_bounds__(64) void k_density(const RunDev* __restrict__ runs,
     const int K = lm.K, k = lm.k;
     const int per = CH / K;   // lane candidates per chunk
     const int chk = per * K;  // staged entries per chunk (a multiple of K)
-    const int kern = R.kernel;
+    const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
     const float H2 = R.H2;
     const float gam = R.gamma;
     const bool keep = masks_fit(nt, total, lm);
@@ -679,9 +679,9 @@ __device__ __forceinline__ void force_pair(const float4& o0, const float4& o1, c
     const float w = vr * invR;
     const float vs = o1.w + q1.w - beta * fminf(w, 0.0f);
     A.vmax = fmaxf(A.vmax, vs);
-    if (alpha > 0.0f && vr < 0.0f) {
+    if (alpha > 0.0f) {  // uniform per run; receding pairs get mu = 0, hence Pi = 0
         const float hb = 0.5f * (o2.x + q2.x);
-        const float mu = hb * vr * fast_rcp(r2 + 0.01f * hb * hb);
+        const float mu = hb * fminf(vr, 0.0f) * fast_rcp(r2 + 0.01f * hb * hb);
         const float cb = 0.5f * (o1.w + q1.w);
         const float rb = 0.5f * (o2.w + q2.w);
         const float Pi = (beta * mu - alpha * cb) * mu * fast_rcp(rb);
@@ -792,7 +792,7 @@ __device__ __forceinline__ void force_finish(const RunDev& R, WorkBuf& W, const 
     }
 }
 
-template <int CH, bool MIXED, bool MHD>
+template <int CH, bool MIXED, bool MHD, int KT>
 __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
                                                int nblk_total, StateBuf S, WorkBuf W, int kmax, int balance)
 {
@@ -815,7 +815,7 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
     const int K = lm.K, k = lm.k;
     const int per = CH / K;
     const int chk = per * K;
-    const int kern = R.kernel;
+    const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
     const float H2 = R.H2;
     const float alpha = R.alpha, beta = R.beta;
     const bool mhd = MHD && R.mhd;
@@ -940,7 +940,7 @@ __device__ __forceinline__ uint32_t mask_bits(const uint32_t* row, int lo, int l
     return lim == 32 ? a : (a & ((1u << lim) - 1u));
 }
 
-template <int CAPD, bool MIXED, bool MHD>
+template <int CAPD, bool MIXED, bool MHD, int KT>
 __global__ __launch_bounds__(64) void k_density_cell(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
                                                      int nblk_total, StateBuf S, WorkBuf W, int kmax)
 {
@@ -960,7 +960,7 @@ __global__ __launch_bounds__(64) void k_density_cell(const RunDev* __restrict__ 
     const int K = lm.K, k = lm.k;
     const int Lr = (total + K - 1) / K;
     const bool keep = nt <= lm.tpg && Lr <= 32 * kMaskWords;
-    const int kern = R.kernel;
+    const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
     const float H2 = R.H2;
     const float gam = R.gamma;
     MaskWriter mw;
@@ -1031,7 +1031,7 @@ __global__ __launch_bounds__(64) void k_density_cell(const RunDev* __restrict__ 
     }
 }
 
-template <int CH, bool MIXED, bool MHD>
+template <int CH, bool MIXED, bool MHD, int KT>
 __global__ __launch_bounds__(64) void k_forces_cell(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
                                                     int nblk_total, StateBuf S, WorkBuf W, int kmax)
 {
@@ -1053,7 +1053,7 @@ __global__ __launch_bounds__(64) void k_forces_cell(const RunDev* __restrict__ r
     const int K = lm.K, k = lm.k;
     const int Lr = (total + K - 1) / K;
     const bool keep = nt <= lm.tpg && Lr <= 32 * kMaskWords;  // the density pass stored masks
-    const int kern = R.kernel;
+    const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
     const float H2 = R.H2;
     const float alpha = R.alpha, beta = R.beta;
     const bool mhd = MHD && R.mhd;
@@ -1348,52 +1348,47 @@ hipError_t launch_scatter_gather(const LaunchCtx& c, const StateBuf& Snew)
     return hipGetLastError();
 }
 
+// Instantiate a pair kernel for (precision, MHD fields, smoothing kernel): KT = -1 reads the kernel
+// type per run (batches that mix kernel types).
+#define SPH_PAIR_DISPATCH(KERNEL, SIZE, ...)                                                        \
+    do {                                                                                            \
+        auto go = [&](auto mixed, auto mhd, auto kt) {                                              \
+            KERNEL<SIZE, decltype(mixed)::value, decltype(mhd)::value, decltype(kt)::value>          \
+                <<<nblk, 64, 0, c.stream>>>(__VA_ARGS__);                                            \
+        };                                                                                          \
+        auto kt_go = [&](auto mixed, auto mhd) {                                                    \
+            switch (c.kern) {                                                                       \
+            case 0: go(mixed, mhd, std::integral_constant<int, 0>{}); break;                        \
+            case 1: go(mixed, mhd, std::integral_constant<int, 1>{}); break;                        \
+            case 2: go(mixed, mhd, std::integral_constant<int, 2>{}); break;                        \
+            default: go(mixed, mhd, std::integral_constant<int, -1>{}); break;                      \
+            }                                                                                       \
+        };                                                                                          \
+        if (c.mixed) {                                                                              \
+            if (c.has_mhd) kt_go(std::true_type{}, std::true_type{});                               \
+            else kt_go(std::true_type{}, std::false_type{});                                        \
+        } else {                                                                                    \
+            if (c.has_mhd) kt_go(std::false_type{}, std::true_type{});                              \
+            else kt_go(std::false_type{}, std::false_type{});                                       \
+        }                                                                                           \
+    } while (0)
+
 template <int CH>
 static void density_ch(const LaunchCtx& c, int nblk)
 {
-    if (c.mixed) {
-        if (c.has_mhd)
-            k_density<CH, true, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
-        else
-            k_density<CH, true, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
-    } else {
-        if (c.has_mhd)
-            k_density<CH, false, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
-        else
-            k_density<CH, false, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
-    }
+    SPH_PAIR_DISPATCH(k_density, CH, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
 }
 
 template <int CAPD>
 static void density_cell_ch(const LaunchCtx& c, int nblk)
 {
-    if (c.mixed) {
-        if (c.has_mhd)
-            k_density_cell<CAPD, true, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
-        else
-            k_density_cell<CAPD, true, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
-    } else {
-        if (c.has_mhd)
-            k_density_cell<CAPD, false, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
-        else
-            k_density_cell<CAPD, false, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
-    }
+    SPH_PAIR_DISPATCH(k_density_cell, CAPD, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
 }
 
 template <int CH>
 static void forces_cell_ch(const LaunchCtx& c, int nblk)
 {
-    if (c.mixed) {
-        if (c.has_mhd)
-            k_forces_cell<CH, true, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
-        else
-            k_forces_cell<CH, true, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
-    } else {
-        if (c.has_mhd)
-            k_forces_cell<CH, false, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
-        else
-            k_forces_cell<CH, false, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
-    }
+    SPH_PAIR_DISPATCH(k_forces_cell, CH, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
 }
 
 hipError_t launch_density(const LaunchCtx& c)
@@ -1415,17 +1410,7 @@ hipError_t launch_density(const LaunchCtx& c)
 template <int CH>
 static void forces_ch(const LaunchCtx& c, int nblk)
 {
-    if (c.mixed) {
-        if (c.has_mhd)
-            k_forces<CH, true, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
-        else
-            k_forces<CH, true, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
-    } else {
-        if (c.has_mhd)
-            k_forces<CH, false, true><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
-        else
-            k_forces<CH, false, false><<<nblk, 64, 0, c.stream>>>(c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
-    }
+    SPH_PAIR_DISPATCH(k_forces, CH, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
 }
 
 hipError_t launch_forces(const LaunchCtx& c)

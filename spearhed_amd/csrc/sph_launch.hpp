@@ -18,6 +18,7 @@ struct LaunchCtx {
     int force_chunk;
     int kmax;     // max lanes per target in the pair passes
     int balance;  // share a target's accepted pairs among its K lanes
+    int kern;     // smoothing kernel of every run (SPH_KERNEL_*), or -1 when runs differ
     StateBuf S;          // current buffer set
     WorkBuf W;
     hipStream_t stream;
