@@ -648,11 +648,11 @@ int sph_set_tuning(sph_ctx* c, const sph_tuning* t)
     const int dc = t->density_chunk, fc = t->force_chunk;
     if (t->pair_balance < 0 || t->pair_balance > 2) return fail(c, SPH_E_INVALID, "pair_balance must be 0, 1 or 2");
     if (t->pair_balance == 2) {
-        if (dc != 512 && dc != 1024) return fail(c, SPH_E_INVALID, "whole-cell density_chunk must be 512 or 1024");
-        if (fc != 128 && fc != 256) return fail(c, SPH_E_INVALID, "whole-cell force_chunk must be 128 or 256");
+        if (dc != 512) return fail(c, SPH_E_INVALID, "whole-cell density_chunk must be 512");
+        if (fc != 128) return fail(c, SPH_E_INVALID, "whole-cell force_chunk must be 128");
     } else {
-        if (dc != 128 && dc != 256 && dc != 512) return fail(c, SPH_E_INVALID, "density_chunk must be 128, 256 or 512");
-        if (fc != 64 && fc != 128 && fc != 256) return fail(c, SPH_E_INVALID, "force_chunk must be 64, 128 or 256");
+        if (dc != 128 && dc != 256) return fail(c, SPH_E_INVALID, "density_chunk must be 128 or 256");
+        if (fc != 64 && fc != 128) return fail(c, SPH_E_INVALID, "force_chunk must be 64 or 128");
     }
     if (t->max_lanes_per_target < 1 || t->max_lanes_per_target > 64)
         return fail(c, SPH_E_INVALID, "max_lanes_per_target must be in [1, 64]");
@@ -1044,9 +1044,7 @@ int sph_autotune(sph_ctx* c, int32_t n_steps, sph_tuning* best)
         for (int fc : {64, 128})
             for (int k : {16, 64})
                 for (int bal : {0, 1}) cands.push_back({dc, fc, t0.waves_per_cu, k, bal});
-    for (int dc : {512, 1024})
-        for (int fc : {128, 256})
-            for (int k : {16, 64}) cands.push_back({dc, fc, t0.waves_per_cu, k, 2});
+    for (int k : {16, 64}) cands.push_back({512, 128, t0.waves_per_cu, k, 2});
     const float cfs[2] = {1.0f, 1.15f};
     int rc = SPH_OK;
     for (float cf : cfs) {

@@ -532,7 +532,7 @@ __device__ __forceinline__ void density_finish(const RunDev& R, int r, StateBuf&
 // density pass: cull of the 27-supercell candidates staged in LDS, density sum, candidate masks,
 // and the per-particle interaction records of the force pass
 // ------------------------------------------------------------------------------------------------
-template <int CH, bool MIXED, bool MHD, int KT>
+template <int CH, bool MIXED, bool MHD, int KT, bool BAL>
 __global__ __launch_bounds__(64) void k_density(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
                                                 int nblk_total, StateBuf S, WorkBuf W, int kmax, int balance)
 {
@@ -616,7 +616,7 @@ __global__ __launch_bounds__(64) void k_density(const RunDev* __restrict__ runs,
                 if (!tgt) msk = 0;
                 if (keep) mw.put(msk, cbase + sb, lim);
                 cnt += (uint32_t)__popc(msk);
-                if (K == 1 || !balance) {
+                if (!BAL || K == 1) {
                     while (msk) {
                         const int ii = __builtin_ctz(msk);
                         msk &= msk - 1;
@@ -792,7 +792,7 @@ __device__ __forceinline__ void force_finish(const RunDev& R, WorkBuf& W, const 
     }
 }
 
-template <int CH, bool MIXED, bool MHD, int KT>
+template <int CH, bool MIXED, bool MHD, int KT, bool BAL>
 __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
                                                int nblk_total, StateBuf S, WorkBuf W, int kmax, int balance)
 {
@@ -872,7 +872,7 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
                     }
                 }
                 if (!tgt) msk = 0;
-                if (K == 1 || !balance) {
+                if (!BAL || K == 1) {
                     while (msk) {
                         const int ii = __builtin_ctz(msk);
                         msk &= msk - 1;
@@ -1348,12 +1348,13 @@ hipError_t launch_scatter_gather(const LaunchCtx& c, const StateBuf& Snew)
     return hipGetLastError();
 }
 
+#define SPH_COMMA ,
 // Instantiate a pair kernel for (precision, MHD fields, smoothing kernel): KT = -1 reads the kernel
 // type per run (batches that mix kernel types).
-#define SPH_PAIR_DISPATCH(KERNEL, SIZE, ...)                                                        \
+#define SPH_PAIR_DISPATCH(KERNEL, SIZE, EXTRA, ...)                                                 \
     do {                                                                                            \
         auto go = [&](auto mixed, auto mhd, auto kt) {                                              \
-            KERNEL<SIZE, decltype(mixed)::value, decltype(mhd)::value, decltype(kt)::value>          \
+            KERNEL<SIZE, decltype(mixed)::value, decltype(mhd)::value, decltype(kt)::value EXTRA>    \
                 <<<nblk, 64, 0, c.stream>>>(__VA_ARGS__);                                            \
         };                                                                                          \
         auto kt_go = [&](auto mixed, auto mhd) {                                                    \
@@ -1373,58 +1374,56 @@ hipError_t launch_scatter_gather(const LaunchCtx& c, const StateBuf& Snew)
         }                                                                                           \
     } while (0)
 
-template <int CH>
+template <int CH, bool BAL>
 static void density_ch(const LaunchCtx& c, int nblk)
 {
-    SPH_PAIR_DISPATCH(k_density, CH, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
+    SPH_PAIR_DISPATCH(k_density, CH, SPH_COMMA BAL, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
 }
 
 template <int CAPD>
 static void density_cell_ch(const LaunchCtx& c, int nblk)
 {
-    SPH_PAIR_DISPATCH(k_density_cell, CAPD, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
+    SPH_PAIR_DISPATCH(k_density_cell, CAPD, , c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
 }
 
 template <int CH>
 static void forces_cell_ch(const LaunchCtx& c, int nblk)
 {
-    SPH_PAIR_DISPATCH(k_forces_cell, CH, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
+    SPH_PAIR_DISPATCH(k_forces_cell, CH, , c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
 }
 
 hipError_t launch_density(const LaunchCtx& c)
 {
     const int nblk = nblk_xcd(c.ncell_tot);
     if (c.balance == 2) {
-        if (c.density_chunk >= 1024) density_cell_ch<1024>(c, nblk);
-        else density_cell_ch<512>(c, nblk);
-        return hipGetLastError();
-    }
-    switch (c.density_chunk) {
-    case 128: density_ch<128>(c, nblk); break;
-    case 512: density_ch<512>(c, nblk); break;
-    default: density_ch<256>(c, nblk); break;
+        density_cell_ch<512>(c, nblk);
+    } else if (c.density_chunk >= 256) {
+        if (c.balance) density_ch<256, true>(c, nblk);
+        else density_ch<256, false>(c, nblk);
+    } else {
+        if (c.balance) density_ch<128, true>(c, nblk);
+        else density_ch<128, false>(c, nblk);
     }
     return hipGetLastError();
 }
 
-template <int CH>
+template <int CH, bool BAL>
 static void forces_ch(const LaunchCtx& c, int nblk)
 {
-    SPH_PAIR_DISPATCH(k_forces, CH, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
+    SPH_PAIR_DISPATCH(k_forces, CH, SPH_COMMA BAL, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
 }
 
 hipError_t launch_forces(const LaunchCtx& c)
 {
     const int nblk = nblk_xcd(c.ncell_tot);
     if (c.balance == 2) {
-        if (c.force_chunk >= 256) forces_cell_ch<256>(c, nblk);
-        else forces_cell_ch<128>(c, nblk);
-        return hipGetLastError();
-    }
-    switch (c.force_chunk) {
-    case 64: forces_ch<64>(c, nblk); break;
-    case 256: forces_ch<256>(c, nblk); break;
-    default: forces_ch<128>(c, nblk); break;
+        forces_cell_ch<128>(c, nblk);
+    } else if (c.force_chunk >= 128) {
+        if (c.balance) forces_ch<128, true>(c, nblk);
+        else forces_ch<128, false>(c, nblk);
+    } else {
+        if (c.balance) forces_ch<64, true>(c, nblk);
+        else forces_ch<64, false>(c, nblk);
     }
     return hipGetLastError();
 }

@@ -224,7 +224,8 @@ MODES = {
     "chunked": dict(density_chunk=128, force_chunk=64, max_lanes_per_target=64, pair_balance=0),
     "balanced": dict(density_chunk=256, force_chunk=128, max_lanes_per_target=16, pair_balance=1),
     "cell512": dict(density_chunk=512, force_chunk=128, max_lanes_per_target=64, pair_balance=2),
-    "cell1024": dict(density_chunk=1024, force_chunk=256, max_lanes_per_target=16, pair_balance=2),
+    "cell512k16": dict(density_chunk=512, force_chunk=128, max_lanes_per_target=16, pair_balance=2),
+    "chunked128": dict(density_chunk=256, force_chunk=128, max_lanes_per_target=16, pair_balance=0),
 }
 
 
