@@ -133,7 +133,10 @@ typedef struct sph_tuning {
     int32_t force_chunk;    /* neighbour records staged in LDS per chunk by the force pass: 64|128|256 */
     int32_t waves_per_cu;   /* persistent-grid waves per CU: 4..32 */
     int32_t max_lanes_per_target; /* cap on K, the lanes that share one target's candidates: 1..64 */
-    int32_t pair_balance;   /* 1: compact each target's accepted pairs and share them among its K lanes */
+    int32_t pair_balance;   /* how the accepted pairs of a target are spread over its K lanes:
+                               0: each lane its own candidates; 1: compacted in LDS and shared
+                               round-robin; 2: whole-supercell neighbour lists (density_chunk 512,
+                               force_chunk 128) */
 } sph_tuning;
 
 typedef struct sph_ctx sph_ctx;

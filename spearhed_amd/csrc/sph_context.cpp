@@ -1045,6 +1045,7 @@ int sph_autotune(sph_ctx* c, int32_t n_steps, sph_tuning* best)
             for (int k : {16, 64})
                 for (int bal : {0, 1}) cands.push_back({dc, fc, t0.waves_per_cu, k, bal});
     for (int k : {16, 64}) cands.push_back({512, 128, t0.waves_per_cu, k, 2});
+
     const float cfs[2] = {1.0f, 1.15f};
     int rc = SPH_OK;
     for (float cf : cfs) {

@@ -539,7 +539,7 @@ __global__ __launch_bounds__(64) void k_density(const RunDev* __restrict__ runs,
     __shared__ NbList L;
     __shared__ float4 sp[CH];
     using LT = typename std::conditional<(CH <= 256), uint8_t, uint16_t>::type;
-    __shared__ LT lst[64 * 32];
+    __shared__ LT lst[BAL ? 64 * 32 : 1];  // compaction lists of the balanced pair loop
     const int cell = xcd_cell(blockIdx.x, nblk_total, ncell_tot);
     if (cell < 0) return;
     const int t0 = (int)W.cs[cell];
@@ -792,7 +792,7 @@ __device__ __forceinline__ void force_finish(const RunDev& R, WorkBuf& W, const 
     }
 }
 
-template <int CH, bool MIXED, bool MHD, int KT, bool BAL>
+template <int CH, bool MIXED, bool MHD, int KT, int BAL>
 __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
                                                int nblk_total, StateBuf S, WorkBuf W, int kmax, int balance)
 {
@@ -800,7 +800,7 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
     __shared__ NbList L;
     __shared__ float4 sp[NP][CH];
     using LT = typename std::conditional<(CH <= 256), uint8_t, uint16_t>::type;
-    __shared__ LT lst[64 * 32];
+    __shared__ LT lst[BAL == 1 ? 64 * 32 : 1];  // compaction lists of the balanced pair loop
     const int cell = xcd_cell(blockIdx.x, nblk_total, ncell_tot);
     if (cell < 0) return;
     const int t0 = (int)W.cs[cell];
@@ -872,7 +872,7 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
                     }
                 }
                 if (!tgt) msk = 0;
-                if (!BAL || K == 1) {
+                if (BAL == 0 || K == 1) {
                     while (msk) {
                         const int ii = __builtin_ctz(msk);
                         msk &= msk - 1;
@@ -1398,16 +1398,16 @@ hipError_t launch_density(const LaunchCtx& c)
     if (c.balance == 2) {
         density_cell_ch<512>(c, nblk);
     } else if (c.density_chunk >= 256) {
-        if (c.balance) density_ch<256, true>(c, nblk);
+        if (c.balance == 1) density_ch<256, true>(c, nblk);
         else density_ch<256, false>(c, nblk);
     } else {
-        if (c.balance) density_ch<128, true>(c, nblk);
+        if (c.balance == 1) density_ch<128, true>(c, nblk);
         else density_ch<128, false>(c, nblk);
     }
     return hipGetLastError();
 }
 
-template <int CH, bool BAL>
+template <int CH, int BAL>
 static void forces_ch(const LaunchCtx& c, int nblk)
 {
     SPH_PAIR_DISPATCH(k_forces, CH, SPH_COMMA BAL, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
@@ -1419,11 +1419,11 @@ hipError_t launch_forces(const LaunchCtx& c)
     if (c.balance == 2) {
         forces_cell_ch<128>(c, nblk);
     } else if (c.force_chunk >= 128) {
-        if (c.balance) forces_ch<128, true>(c, nblk);
-        else forces_ch<128, false>(c, nblk);
+        if (c.balance == 1) forces_ch<128, 1>(c, nblk);
+        else forces_ch<128, 0>(c, nblk);
     } else {
-        if (c.balance) forces_ch<64, true>(c, nblk);
-        else forces_ch<64, false>(c, nblk);
+        if (c.balance == 1) forces_ch<64, 1>(c, nblk);
+        else forces_ch<64, 0>(c, nblk);
     }
     return hipGetLastError();
 }
