@@ -139,6 +139,39 @@ def run_ensemble(args):
             "speedup_vs_plain": batched / plain}
 
 
+def run_serving(args):
+    """Continuous batching of a Poisson stream of independent runs (Brio-Wu, rotor, Orszag-Tang drawn
+    uniformly; sizes log-uniform in [8k, 1M]; 20 steps each), admitted by HBM budget. Initial
+    conditions are generated before the clock starts; uploads and priming of admitted runs happen
+    between batched steps and are reported separately from the step latency."""
+    from spearhed_amd.serving import Request, Server, poisson_arrivals
+
+    rng = np.random.default_rng(args.serve_seed)
+    names = sorted(ic.SERVING_PROBLEMS)
+    arrivals = poisson_arrivals(args.serve_requests, args.serve_rate, seed=args.serve_seed)
+    reqs = []
+    for r in range(args.serve_requests):
+        n = int(round(math.exp(rng.uniform(math.log(8_000), math.log(1_000_000)))))
+        cfg, ps = ic.SERVING_PROBLEMS[names[int(rng.integers(len(names)))]](n, args.serve_seed * 1000 + r)
+        reqs.append(Request(r, cfg, ps, 20, arrival=float(arrivals[r])))
+    srv = Server(int(args.serve_budget_gb * 2 ** 30), precision="mixed", max_runs=256)
+    try:
+        t0 = time.perf_counter()
+        summary = srv.serve(reqs, realtime=True)
+        wall = time.perf_counter() - t0
+        lat = np.array([res.finished - res.arrival for res in srv.results.values()]) * 1e3
+    finally:
+        srv.close()
+    work = sum(q.ps.n * q.steps for q in reqs)
+    return {"requests": len(reqs), "rate_per_s": args.serve_rate, "hbm_budget_GB": args.serve_budget_gb,
+            "particles_total": int(sum(q.ps.n for q in reqs)), "step_ms_p50": summary["step_ms_p50"],
+            "step_ms_p99": summary["step_ms_p99"], "mean_batch_runs": summary["mean_batch_runs"],
+            "particle_steps_per_s_in_steps": summary["particle_steps_per_s"],
+            "particle_steps_per_s_wall": work / wall, "request_latency_ms_p50": float(np.percentile(lat, 50)),
+            "request_latency_ms_p99": float(np.percentile(lat, 99)),
+            "admission_and_upload_ms_total": summary["admission_ms_total"], "wall_s": wall}
+
+
 def fp64_error(cfg, ps, g):
     """Error of the GPU's first density and force pass against the FP64 oracle on the same input:
     density relative, derivatives relative to the sum of |pair terms| (the FP32 summation bound)."""
@@ -201,6 +234,11 @@ def main():
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--no-tune-ensemble", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-budget-gb", type=float, default=4.0)
+    ap.add_argument("--serve-seed", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the step would shard by "
@@ -253,6 +291,8 @@ def main():
     }
     if not args.no_ensemble:
         out["ensemble"] = run_ensemble(args)
+    if not args.no_serve:
+        out["serving"] = run_serving(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m["cfg"], m["ps"], args.cpu_seconds)
         out["fp64_oracle_error"] = fp64_error(m["cfg"], m["ps"], m["primed"])

@@ -154,8 +154,8 @@ int sph_ctx_destroy(sph_ctx* ctx);
 int sph_pool_create(int64_t max_particles, int64_t max_cells, int32_t max_runs, int32_t precision, int32_t mhd,
                     void* stream, sph_ctx** out);
 /* New run set = the kept runs (old indices, in the given order; their state and dt/time move with
- * them) followed by the added runs (to be filled with sph_upload and given a first dt by
- * sph_prime, which leaves the kept runs' dt untouched). */
+ * them) followed by the added runs, to be filled with sph_upload. An added run's first step is its
+ * priming step (see sph_upload); kept runs step on undisturbed. */
 int sph_pool_set_runs(sph_ctx* ctx, const int32_t* keep, int32_t n_keep, const sph_run_config* add,
                       const int64_t* add_sizes, int32_t n_add);
 const char* sph_last_error(const sph_ctx* ctx);
@@ -172,6 +172,9 @@ int64_t sph_run_cells(const sph_run_config* cfg);
 int64_t sph_pool_bytes(int64_t max_particles, int64_t max_cells, int32_t max_runs, int32_t precision, int32_t mhd);
 
 /* ---- data movement (the caller owns the host arrays) ---------------------------------------- */
+/* New initial data for a run: its time restarts at 0 and its next step is a priming step (density,
+ * forces and its first CFL dt, with no drift, no kick and no time advance: the arithmetic of
+ * sph_prime for that run alone). Particle ids must be unique within the run. */
 int sph_upload(sph_ctx* ctx, int32_t run, const sph_host_soa* src);
 /* Downloads in the context's current (supercell-sorted) order; compare by id, never by position. */
 int sph_download(sph_ctx* ctx, int32_t run, sph_host_soa* dst);

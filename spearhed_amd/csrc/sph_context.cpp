@@ -712,10 +712,26 @@ int sph_upload(sph_ctx* c, int32_t run, const sph_host_soa* s)
         !s->smoothing_length || !s->internal_energy)
         return fail(c, SPH_E_INVALID, "upload: id, pos, mass, vel, smoothing_length, internal_energy are required");
     {
-        std::vector<uint64_t> ids(s->id, s->id + s->n);
-        std::sort(ids.begin(), ids.end());
-        if (std::adjacent_find(ids.begin(), ids.end()) != ids.end())
-            return fail(c, SPH_E_INVALID, "upload: particle ids must be unique within a run");
+        // ids must be unique within a run (the re-sort ranks particles of a supercell by id):
+        // a bitmap when the ids are dense (the usual 0..n-1), a sort otherwise
+        const uint64_t* id = s->id;
+        const int64_t n = s->n;
+        const uint64_t mx = n ? *std::max_element(id, id + n) : 0;
+        bool dup = false;
+        if (mx < (uint64_t)(4 * n + 64)) {
+            std::vector<uint64_t> seen((size_t)(mx / 64 + 1), 0);
+            for (int64_t i = 0; i < n && !dup; ++i) {
+                const uint64_t bit = 1ull << (id[i] & 63);
+                uint64_t& w = seen[(size_t)(id[i] >> 6)];
+                dup = (w & bit) != 0;
+                w |= bit;
+            }
+        } else {
+            std::vector<uint64_t> ids(id, id + n);
+            std::sort(ids.begin(), ids.end());
+            dup = std::adjacent_find(ids.begin(), ids.end()) != ids.end();
+        }
+        if (dup) return fail(c, SPH_E_INVALID, "upload: particle ids must be unique within a run");
     }
     for (int64_t i = 0; i < s->n; ++i)
         if (!(s->smoothing_length[i] > 0.0f)) return fail(c, SPH_E_INVALID, "upload: smoothing_length must be > 0");

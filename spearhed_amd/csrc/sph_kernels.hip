@@ -93,7 +93,8 @@ __global__ __launch_bounds__(256) void k_drift_bin(const RunDev* __restrict__ ru
     if (active) {
         const double dt = dt_host > 0.0 ? dt_host : W.dt[r];
         const uint8_t live = S.mask[i];
-        const bool mov = do_drift && is_movable(S.species[i], live);
+        // a run whose next step primes it (fresh initial data) does not move in that step
+        const bool mov = do_drift && is_movable(S.species[i], live) && !W.prime_flag[r];
         int c[3];
         for (int a = 0; a < 3; ++a) {
 #pragma clang fp contract(off)
@@ -1183,7 +1184,8 @@ __global__ __launch_bounds__(256) void k_kick(const RunDev* __restrict__ runs, S
             float u = S.u[i];
             const float ax = W.dvdt[0][i], ay = W.dvdt[1][i], az = W.dvdt[2][i];
             float bx = 0.f, by = 0.f, bz = 0.f;
-            if (do_kick) {
+            const bool kick = do_kick && !W.prime_flag[r];
+            if (kick) {
                 vx = vx + ax * dtf;
                 vy = vy + ay * dtf;
                 vz = vz + az * dtf;
@@ -1197,7 +1199,7 @@ __global__ __launch_bounds__(256) void k_kick(const RunDev* __restrict__ runs, S
                 bx = S.B[0][i];
                 by = S.B[1][i];
                 bz = S.B[2][i];
-                if (do_kick) {
+                if (kick) {
                     bx = bx + W.dbdt[0][i] * dtf;
                     by = by + W.dbdt[1][i] * dtf;
                     bz = bz + W.dbdt[2][i] * dtf;
@@ -1282,8 +1284,9 @@ __global__ __launch_bounds__(256) void k_finalize(const RunDev* __restrict__ run
     }
     if (threadIdx.x == 0) {
         const double dt_used = dt_host > 0.0 ? dt_host : W.dt[r];
+        const int priming = W.prime_flag[r];
         double t = W.time[r];
-        if (advance) t += dt_used;
+        if (advance && !priming) t += dt_used;
         W.time[r] = t;
         const double dtn = R.dt_fixed > 0.0 ? R.dt_fixed : red[0][0];
         double* c = W.cons + (int64_t)r * 8;
@@ -1296,8 +1299,11 @@ __global__ __launch_bounds__(256) void k_finalize(const RunDev* __restrict__ run
         c[6] = t;
         c[7] = (double)__uint_as_float(W.chbits[r]);
         // a step sets the next dt; a reduction outside a step (prime, queries) sets it only for
-        // runs still waiting for their first dt, so runs already stepping keep their own sequence
-        if (advance || W.prime_flag[r]) W.dt[r] = dtn;
+        // runs still waiting for their first dt, so runs already stepping keep their own sequence.
+        // A run's first step after new initial data is its priming step: no drift, no kick, no
+        // time advance (the same arithmetic as sph_prime), only its first CFL dt
+        if (advance || priming) W.dt[r] = dtn;
+        if (advance) W.prime_flag[r] = 0;  // the step that primed the run is done
     }
 }
 

@@ -161,11 +161,11 @@ class Server:
         if not finished and not add:
             return
         new_idx = self.pool.set_runs(keep, [(q.cfg, q.ps.n) for q in add])
-        self.resident = [self.resident[i] for i in keep] + [_Resident(q, 0, now) for q in add]
+        # an admitted run's first batched step primes it (density, forces, first dt; no motion),
+        # so it needs one step more than it asked for and no separate priming pass
+        self.resident = [self.resident[i] for i in keep] + [_Resident(q, -1, now) for q in add]
         for j, q in zip(new_idx, add):
             self.pool.upload(j, q.ps)
-        if add:
-            self.pool.prime()
 
     def serve(self, requests: Optional[List[Request]] = None, max_steps: int = 1 << 30,
               realtime: bool = True) -> dict:

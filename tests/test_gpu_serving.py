@@ -134,3 +134,25 @@ def test_many_runs_keep_their_tables():
         ctx.sync()
         means = [ctx.neighbour_total(r) / p.n for r, (_, p) in enumerate(runs)]
     assert min(means) > 40, means
+
+
+@pytest.mark.parametrize("dt_fixed", [0.0, 2e-4])
+def test_first_step_after_upload_primes(dt_fixed):
+    """After new initial data the run's first step is its priming step (no drift, no kick, no time
+    advance; first CFL dt), so upload + (k+1) steps equals upload + prime + k steps bit for bit."""
+    cfg, ps = ic.kelvin_helmholtz(5000, 77)
+    cfg.dt_fixed = dt_fixed
+    outs = []
+    for primed in (True, False):
+        with Context([cfg], [ps.n], precision="mixed") as ctx:
+            ctx.upload(0, ps)
+            if primed:
+                ctx.prime()
+                ctx.step(3)
+            else:
+                ctx.step(4)
+            ctx.sync()
+            outs.append((ctx.download(0).sorted_by_id(), ctx.conserved()[0]))
+    (a, ca), (b, cb) = outs
+    _same(a, b)
+    assert ca == cb

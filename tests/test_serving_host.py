@@ -85,8 +85,9 @@ def test_fifo_and_retire_order():
     srv.serve(reqs, realtime=False)
     admits = [c for c in srv.pool.calls if c[0] == "set_runs"]
     assert admits[0] == ("set_runs", [], 4)           # all fit: admitted together
-    assert admits[1] == ("set_runs", [1, 3], 0)       # after 2 steps runs 0 and 2 retire
-    assert srv.pool.steps == 4
+    assert admits[1] == ("set_runs", [1, 3], 0)       # after their priming step + 2 steps runs 0, 2 retire
+    assert srv.pool.steps == 5                        # the longest request: priming step + 4 steps
+    assert ("prime",) not in srv.pool.calls           # admitted runs are primed by their first step
     assert srv.results[0].ps[1] == 0 and srv.results[2].ps[1] == 2  # downloaded before the move
 
 
