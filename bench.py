@@ -146,15 +146,24 @@ def run_serving(args):
     between batched steps and are reported separately from the step latency."""
     from spearhed_amd.serving import Request, Server, poisson_arrivals
 
+    from spearhed_amd import setups
+
     rng = np.random.default_rng(args.serve_seed)
-    names = sorted(ic.SERVING_PROBLEMS)
+    names = sorted(setups.DEVICE_SERVING_PROBLEMS)
     arrivals = poisson_arrivals(args.serve_requests, args.serve_rate, seed=args.serve_seed)
     reqs = []
     for r in range(args.serve_requests):
         n = int(round(math.exp(rng.uniform(math.log(8_000), math.log(1_000_000)))))
-        cfg, ps = ic.SERVING_PROBLEMS[names[int(rng.integers(len(names)))]](n, args.serve_seed * 1000 + r)
-        reqs.append(Request(r, cfg, ps, 20, arrival=float(arrivals[r])))
-    srv = Server(int(args.serve_budget_gb * 2 ** 30), precision="mixed", max_runs=256)
+        name = names[int(rng.integers(len(names)))]
+        if args.serve_host_ic:  # particles generated on the host and uploaded
+            cfg, ps = ic.SERVING_PROBLEMS[name](n, args.serve_seed * 1000 + r)
+            reqs.append(Request(r, cfg, ps, 20, arrival=float(arrivals[r])))
+        else:  # lattice blocks filled on the device
+            cfg, blocks, _ = setups.DEVICE_SERVING_PROBLEMS[name](n)
+            reqs.append(Request(r, cfg, None, 20, arrival=float(arrivals[r]), blocks=blocks,
+                                seed=args.serve_seed * 1000 + r))
+    srv = Server(int(args.serve_budget_gb * 2 ** 30), precision="mixed", max_runs=256,
+                 response_fields=("id", "pos", "vel", "rho", "u", "B"))
     try:
         t0 = time.perf_counter()
         summary = srv.serve(reqs, realtime=True)
@@ -162,14 +171,19 @@ def run_serving(args):
         lat = np.array([res.finished - res.arrival for res in srv.results.values()]) * 1e3
     finally:
         srv.close()
-    work = sum(q.ps.n * q.steps for q in reqs)
+    work = sum(q.n * q.steps for q in reqs)
     return {"requests": len(reqs), "rate_per_s": args.serve_rate, "hbm_budget_GB": args.serve_budget_gb,
-            "particles_total": int(sum(q.ps.n for q in reqs)), "step_ms_p50": summary["step_ms_p50"],
-            "step_ms_p99": summary["step_ms_p99"], "mean_batch_runs": summary["mean_batch_runs"],
+            "initial_conditions": "host upload" if args.serve_host_ic else "device lattice blocks",
+            "response": "id, pos, vel, rho, u, B + FP64 sums, copied out asynchronously through pinned memory",
+            "particles_total": int(sum(q.n for q in reqs)), "step_ms_p50": summary["step_ms_p50"],
+            "step_ms_p99": summary["step_ms_p99"], "step_wall_ms_p50": summary["step_wall_ms_p50"],
+            "step_wall_ms_p99": summary["step_wall_ms_p99"], "mean_batch_runs": summary["mean_batch_runs"],
             "particle_steps_per_s_in_steps": summary["particle_steps_per_s"],
             "particle_steps_per_s_wall": work / wall, "request_latency_ms_p50": float(np.percentile(lat, 50)),
             "request_latency_ms_p99": float(np.percentile(lat, 99)),
-            "admission_and_upload_ms_total": summary["admission_ms_total"], "wall_s": wall}
+            "admission_and_upload_ms_total": summary["admission_ms_total"],
+            "retire_ms_total": summary["retire_ms_total"], "move_ms_total": summary["move_ms_total"],
+            "fill_ms_total": summary["fill_ms_total"], "wall_s": wall}
 
 
 def fp64_error(cfg, ps, g):
@@ -239,6 +253,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-budget-gb", type=float, default=4.0)
     ap.add_argument("--serve-seed", type=int, default=5)
+    ap.add_argument("--serve-host-ic", action="store_true", help="serve host-generated particles instead of device ICs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the step would shard by "

@@ -171,6 +171,43 @@ int64_t sph_run_cells(const sph_run_config* cfg);
 /* Device bytes sph_pool_create reserves for these capacities; -1 if invalid. Host-only. */
 int64_t sph_pool_bytes(int64_t max_particles, int64_t max_cells, int32_t max_runs, int32_t precision, int32_t mhd);
 
+/* ---- initial conditions on the device --------------------------------------------------------
+ * A run can be filled on the GPU from lattice blocks instead of uploaded: the analogue of the
+ * reference's setup blocks placing particles on a simple cubic lattice
+ * (packages/pmacc/include/spmacc/particles/initialization/SC.hpp:117-207: per-axis counts from the
+ * block extent and spacing, x truncated with round-off tolerance, other axes rounded; particles at
+ * cell centres, x fastest). Each position then gets a uniform jitter of +-jitter * cell edge per
+ * axis from Philox4x32-10 (Salmon et al. 2011) keyed by (seed, particle id), so the result does not
+ * depend on the launch. The setup fills the fields from the position. */
+#define SPH_SETUP_UNIFORM 0     /* density, pressure, velocity, bfield as given */
+#define SPH_SETUP_ROTOR 1       /* params: centre x, centre y, radius, edge speed, disc density:
+                                   inside the disc rho = disc density, v = rigid rotation */
+#define SPH_SETUP_ORSZAG_TANG 2 /* params: v0, B0: v = v0 (-sin 2 pi y, sin 2 pi x, 0),
+                                   B = B0 (-sin 2 pi y, sin 4 pi x, 0) */
+#define SPH_SETUP_SHEAR_LAYER 3 /* params: dV, density ratio, amplitude, phase: rho * ratio and
+                                   vx = +dV/2 for |y - 1/2| < 1/4, else vx = -dV/2;
+                                   vy = amplitude sin(4 pi x + phase) */
+
+typedef struct sph_lattice_block {
+    double lo[3], hi[3];  /* the block's region */
+    double spacing;       /* nominal lattice spacing */
+    double density;       /* background density: mass per particle = density * cell volume */
+    double pressure;      /* u = pressure / ((gamma - 1) density) */
+    float velocity[3];
+    float bfield[3];
+    int32_t species;      /* SPH_SPECIES_* */
+    int32_t setup;        /* SPH_SETUP_* */
+    double params[6];
+} sph_lattice_block;
+
+/* Particles the blocks place (host-only); -1 if a block is invalid. */
+int64_t sph_lattice_sites(const sph_lattice_block* blocks, int32_t n_blocks);
+/* Fill run `run` (whose size must equal the blocks' sites) on the device: ids 0..n-1 in block
+ * order, smoothing length h0 of the run, then the setup's fields. Like an upload, the run restarts
+ * at t = 0 and its next step primes it. */
+int sph_init_run(sph_ctx* ctx, int32_t run, const sph_lattice_block* blocks, int32_t n_blocks, uint64_t seed,
+                 double jitter);
+
 /* ---- data movement (the caller owns the host arrays) ---------------------------------------- */
 /* New initial data for a run: its time restarts at 0 and its next step is a priming step (density,
  * forces and its first CFL dt, with no drift, no kick and no time advance: the arithmetic of
@@ -178,6 +215,17 @@ int64_t sph_pool_bytes(int64_t max_particles, int64_t max_cells, int32_t max_run
 int sph_upload(sph_ctx* ctx, int32_t run, const sph_host_soa* src);
 /* Downloads in the context's current (supercell-sorted) order; compare by id, never by position. */
 int sph_download(sph_ctx* ctx, int32_t run, sph_host_soa* dst);
+/* Copy-out for responses (the step after the path: the reference copies selected species fields
+ * out for output, spmacc/particles/algorithms/CopyParticlesToDynSoA.hpp, include/spearhed/memory.hpp:42-49):
+ * enqueue device-to-host copies of the non-NULL columns of `dst` (and, if `cons` is non-NULL, the
+ * run's FP64 sums and next dt as its last step left them) on the context's stream and return. The
+ * host buffers are valid after the next call that synchronises the stream (sph_sync,
+ * sph_pool_set_runs, sph_download); they should be pinned (sph_host_alloc) for the copies to run
+ * asynchronously. Positions are available in mixed-precision contexts only. */
+int sph_download_async(sph_ctx* ctx, int32_t run, const sph_host_soa* dst, sph_conserved* cons);
+/* Pinned host memory for sph_download_async / sph_upload staging. */
+int sph_host_alloc(int64_t bytes, void** out);
+int sph_host_free(void* p);
 
 /* ---- stages (Simulation::runOneStep order: drift, resort, density, forces, integrate) -------- */
 int sph_stage_drift(sph_ctx* ctx, double dt);      /* dt <= 0: use each run's device-resident CFL dt */

@@ -79,6 +79,37 @@ _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
 
 
+SETUP_UNIFORM, SETUP_ROTOR, SETUP_ORSZAG_TANG, SETUP_SHEAR_LAYER = 0, 1, 2, 3
+
+
+class LatticeBlock(C.Structure):
+    _fields_ = [
+        ("lo", C.c_double * 3),
+        ("hi", C.c_double * 3),
+        ("spacing", C.c_double),
+        ("density", C.c_double),
+        ("pressure", C.c_double),
+        ("velocity", C.c_float * 3),
+        ("bfield", C.c_float * 3),
+        ("species", C.c_int32),
+        ("setup", C.c_int32),
+        ("params", C.c_double * 6),
+    ]
+
+
+def make_block(lo, hi, spacing, density, pressure, velocity=(0.0, 0.0, 0.0), bfield=(0.0, 0.0, 0.0), species=0,
+               setup=SETUP_UNIFORM, params=()) -> LatticeBlock:
+    b = LatticeBlock()
+    b.lo[:] = [float(v) for v in lo]
+    b.hi[:] = [float(v) for v in hi]
+    b.spacing, b.density, b.pressure = float(spacing), float(density), float(pressure)
+    b.velocity[:] = [float(v) for v in velocity]
+    b.bfield[:] = [float(v) for v in bfield]
+    b.species, b.setup = int(species), int(setup)
+    b.params[:] = [float(v) for v in params] + [0.0] * (6 - len(params))
+    return b
+
+
 class HostSoA(C.Structure):
     _fields_ = [
         ("n", C.c_int64),
@@ -179,6 +210,11 @@ PROTOTYPES = {
     "sph_bytes_per_run": (C.c_int64, [C.POINTER(RunConfig), C.c_int64, C.c_int32]),
     "sph_run_cells": (C.c_int64, [C.POINTER(RunConfig)]),
     "sph_pool_bytes": (C.c_int64, [C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
+    "sph_lattice_sites": (C.c_int64, [C.POINTER(LatticeBlock), C.c_int32]),
+    "sph_download_async": (C.c_int, [_CTX, C.c_int32, C.POINTER(HostSoA), C.POINTER(Conserved)]),
+    "sph_host_alloc": (C.c_int, [C.c_int64, C.POINTER(C.c_void_p)]),
+    "sph_host_free": (C.c_int, [C.c_void_p]),
+    "sph_init_run": (C.c_int, [_CTX, C.c_int32, C.POINTER(LatticeBlock), C.c_int32, C.c_uint64, C.c_double]),
     "sph_upload": (C.c_int, [_CTX, C.c_int32, C.POINTER(HostSoA)]),
     "sph_download": (C.c_int, [_CTX, C.c_int32, C.POINTER(HostSoA)]),
     "sph_stage_drift": (C.c_int, [_CTX, C.c_double]),

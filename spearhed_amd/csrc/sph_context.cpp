@@ -14,6 +14,7 @@
 #include "../../include/spearhed_sph.h"
 #include "sph_device.hpp"
 #include "sph_launch.hpp"
+#include "sph_setup.hpp"
 
 using namespace sph;
 
@@ -773,6 +774,99 @@ int sph_upload(sph_ctx* c, int32_t run, const sph_host_soa* s)
     c->sorted = false;
     c->dens_valid = false;
     return SPH_OK;
+}
+
+int64_t sph_lattice_sites(const sph_lattice_block* blocks, int32_t n_blocks)
+{
+    return lattice_sites(blocks, n_blocks);
+}
+
+int sph_init_run(sph_ctx* c, int32_t run, const sph_lattice_block* blocks, int32_t n_blocks, uint64_t seed,
+                 double jitter)
+{
+    if (!c || run < 0 || run >= c->nruns) return SPH_E_INVALID;
+    const int64_t sites = lattice_sites(blocks, n_blocks);
+    if (sites < 0) return fail(c, SPH_E_INVALID, "init_run: invalid lattice block (at most 8 blocks)");
+    const RunDev& R = c->runs_h[run];
+    if (sites != R.np) return fail(c, SPH_E_INVALID, "init_run: the blocks' lattice sites must equal the run's size");
+    if (!(jitter >= 0.0 && jitter < 0.5)) return fail(c, SPH_E_INVALID, "init_run: jitter must be in [0, 0.5)");
+    for (int b = 0; b < n_blocks; ++b)
+        for (int a = 0; a < 3; ++a) {
+            const double lo = R.lo[a], hi = R.lo[a] + R.L[a];
+            if (blocks[b].lo[a] < lo || blocks[b].hi[a] > hi)
+                return fail(c, SPH_E_INVALID, "init_run: a block extends outside the run's box");
+        }
+    const sph_run_config& cfg = c->cfg[run];
+    HIPTRY(c, launch_init_lattice(blocks, n_blocks, R.np, R.p0, seed, jitter, cfg.h0, cfg.gamma, c->mixed ? 1 : 0,
+                                  c->has_mhd ? 1 : 0, c->S[c->cur], c->stream));
+    // as an upload: time restarts at 0 and the next step primes the run
+    const double dt0 = cfg.dt_fixed > 0 ? cfg.dt_fixed : 0.0, t0 = 0.0;
+    const int32_t one = 1;
+    HIPTRY(c, hipMemcpyAsync(c->W.dt + run, &dt0, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, hipMemcpyAsync(c->W.time + run, &t0, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, hipMemcpyAsync(c->W.prime_flag + run, &one, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    if (c->binned) HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_tot, c->stream));
+    c->binned = false;
+    c->sorted = false;
+    c->dens_valid = false;
+    return SPH_OK;
+}
+
+int sph_host_alloc(int64_t bytes, void** out)
+{
+    if (!out || bytes < 1) return SPH_E_INVALID;
+    *out = nullptr;
+    return hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault) == hipSuccess ? SPH_OK : SPH_E_NOMEM;
+}
+
+int sph_host_free(void* p)
+{
+    if (!p) return SPH_OK;
+    return hipHostFree(p) == hipSuccess ? SPH_OK : SPH_E_DEVICE;
+}
+
+int sph_download_async(sph_ctx* c, int32_t run, const sph_host_soa* d, sph_conserved* cons)
+{
+    if (!c || !d || run < 0 || run >= c->nruns) return SPH_E_INVALID;
+    const RunDev& R = c->runs_h[run];
+    if (d->n != R.np) return fail(c, SPH_E_INVALID, "download_async: n must equal the run's size");
+    if (!c->mixed && (d->pos[0] || d->pos[1] || d->pos[2]))
+        return fail(c, SPH_E_INVALID, "download_async: positions need conversion in FP32 contexts; use sph_download");
+    const int64_t n = R.np, o = R.p0;
+    const StateBuf& S = c->S[c->cur];
+    int rc = 0;
+    rc |= fetch_col(c, d->id, S.id + o, n);
+    rc |= fetch_col(c, d->mask, S.mask + o, n);
+    rc |= fetch_col(c, d->species, S.species + o, n);
+    for (int a = 0; a < 3; ++a) {
+        if (c->mixed) rc |= fetch_col(c, d->pos[a], S.pos64[a] + o, n);
+        rc |= fetch_col(c, d->vel[a], S.vel[a] + o, n);
+        rc |= fetch_col(c, d->dvdt[a], c->W.dvdt[a] + o, n);
+        if (c->has_mhd) {
+            rc |= fetch_col(c, d->bfield[a], S.B[a] + o, n);
+            rc |= fetch_col(c, d->dbdt[a], c->W.dbdt[a] + o, n);
+        }
+    }
+    rc |= fetch_col(c, d->mass, S.mass + o, n);
+    rc |= fetch_col(c, d->density, S.rho + o, n);
+    rc |= fetch_col(c, d->smoothing_length, S.h + o, n);
+    rc |= fetch_col(c, d->internal_energy, S.u + o, n);
+    rc |= fetch_col(c, d->dudt, c->W.dudt + o, n);
+    rc |= fetch_col(c, d->vsig, c->W.vsig + o, n);
+    rc |= fetch_col(c, d->n_neighbours, c->W.nnb + o, n);
+    if (c->has_mhd) {
+        rc |= fetch_col(c, d->psi, S.psi + o, n);
+        rc |= fetch_col(c, d->dpsidt, c->W.dpsidt + o, n);
+        rc |= fetch_col(c, d->divb, c->W.divb + o, n);
+    }
+    if (cons) {
+        // the FP64 sums and CFL dt that the run's last step (or reduction) left on the device
+        static_assert(sizeof(sph_conserved) == 8 * sizeof(double), "sph_conserved is 8 doubles");
+        HIPTRY(c, hipMemcpyAsync(cons, c->W.cons + (int64_t)run * 8, sizeof(sph_conserved), hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
+    return rc ? SPH_E_DEVICE : SPH_OK;
 }
 
 int sph_download(sph_ctx* c, int32_t run, sph_host_soa* d)

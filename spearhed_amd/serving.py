@@ -12,8 +12,8 @@ All resident runs then advance one step together (one launch per stage for the w
 run keeps its own dt and time, and its summation orders depend only on its own data, so a run's
 result is bit-identical whether it is served alone or in a mix.
 
-The step latency recorded per batched step is the wall time of the step (host launch to device
-completion); p50 / p99 come from those samples.
+The step latency recorded per batched step is its device time (HIP events on the pool's stream);
+the host wall time from launch to completion is kept beside it. p50 / p99 come from those samples.
 """
 from __future__ import annotations
 
@@ -42,22 +42,37 @@ def run_bytes(cfg: RunConfig, n: int, precision: str = "mixed") -> int:
 
 @dataclass
 class Request:
+    """A run to serve: its configuration and either its particles (uploaded from the host) or lattice
+    blocks (capi.LatticeBlock, filled on the device by sph_init_run with `seed`)."""
     rid: int
     cfg: RunConfig
-    ps: ParticleSet
+    ps: Optional[ParticleSet]
     steps: int
     arrival: float = 0.0  # seconds after the server's start
+    blocks: Optional[list] = None
+    seed: int = 0
+    n: int = field(init=False, default=0)
+
+    def __post_init__(self):
+        if self.ps is not None:
+            self.n = self.ps.n
+        else:
+            arr = (capi.LatticeBlock * len(self.blocks))(*self.blocks)
+            self.n = int(capi.load().sph_lattice_sites(arr, len(self.blocks)))
+            if self.n < 0:
+                raise capi.SphError(f"request {self.rid}: invalid lattice blocks")
 
 
 @dataclass
 class Result:
     rid: int
-    ps: ParticleSet
+    ps: Optional[ParticleSet]  # full final state (full downloads) or None
     conserved: dict
     steps: int
     arrival: float
     admitted: float
     finished: float
+    fields: dict = field(default_factory=dict)  # selected response fields (copy-out path)
 
 
 @dataclass
@@ -73,6 +88,10 @@ class ServeStats:
     batch_runs: List[int] = field(default_factory=list)
     batch_particles: List[int] = field(default_factory=list)
     admit_ms: List[float] = field(default_factory=list)
+    retire_ms: List[float] = field(default_factory=list)   # reductions + downloads of finished runs
+    move_ms: List[float] = field(default_factory=list)     # sph_pool_set_runs
+    fill_ms: List[float] = field(default_factory=list)     # uploads / device fills of admitted runs
+    step_wall_ms: List[float] = field(default_factory=list)  # host launch to completion, per step
 
     def summary(self) -> dict:
         s = np.array(self.step_ms) if self.step_ms else np.zeros(1)
@@ -82,9 +101,14 @@ class ServeStats:
             "steps": len(self.step_ms),
             "step_ms_p50": float(np.percentile(s, 50)),
             "step_ms_p99": float(np.percentile(s, 99)),
+            "step_wall_ms_p50": float(np.percentile(self.step_wall_ms, 50)) if self.step_wall_ms else None,
+            "step_wall_ms_p99": float(np.percentile(self.step_wall_ms, 99)) if self.step_wall_ms else None,
             "mean_batch_runs": float(np.mean(self.batch_runs)) if self.batch_runs else 0.0,
             "particle_steps_per_s": float(ps.sum() / busy) if busy > 0 else 0.0,
             "admission_ms_total": float(np.sum(self.admit_ms)) if self.admit_ms else 0.0,
+            "retire_ms_total": float(np.sum(self.retire_ms)) if self.retire_ms else 0.0,
+            "move_ms_total": float(np.sum(self.move_ms)) if self.move_ms else 0.0,
+            "fill_ms_total": float(np.sum(self.fill_ms)) if self.fill_ms else 0.0,
         }
 
 
@@ -97,7 +121,10 @@ class Server:
     """
 
     def __init__(self, hbm_budget: int, precision: str = "mixed", max_runs: int = 512, mhd: bool = True,
-                 cells_per_particle: float = 0.25, pool_factory=Pool):
+                 cells_per_particle: float = 0.25, pool_factory=Pool, response_fields=None):
+        """response_fields: None returns each run's full final state (synchronous download); a tuple of
+        names (stepper.RESPONSE_FIELDS) returns only those, copied out asynchronously through pinned
+        memory while the pool moves on."""
         self.lib = capi.load()
         self.precision = precision
         self.prec_code = capi.PREC_MIXED if precision == "mixed" else capi.PREC_FP32
@@ -117,16 +144,24 @@ class Server:
         self.stats = ServeStats()
         self.results: dict = {}
         self._t0 = None
+        self.response_fields = tuple(response_fields) if response_fields else None
+        self._ring = None
+        self._pending = []  # (resident, arrays, conserved, finished time) awaiting a synchronisation
+        if self.response_fields:
+            from .stepper import RESPONSE_FIELDS, PinnedRing
+            per = sum(np.dtype(RESPONSE_FIELDS[f][1]).itemsize * RESPONSE_FIELDS[f][2]
+                      for f in self.response_fields)
+            self._ring = PinnedRing(self.max_particles * per + 512 * (len(self.response_fields) + 1) * 256)
 
     # -- bookkeeping -------------------------------------------------------------------------------
     def run_bytes(self, req: Request) -> int:
-        return run_bytes(req.cfg, req.ps.n, self.precision)
+        return run_bytes(req.cfg, req.n, self.precision)
 
     def resident_bytes(self) -> int:
         return sum(self.run_bytes(r.req) for r in self.resident)
 
     def submit(self, req: Request):
-        if (self.run_bytes(req) > self.budget or req.ps.n > self.max_particles
+        if (self.run_bytes(req) > self.budget or req.n > self.max_particles
                 or self.pool.cells_needed(req.cfg) > self.max_cells):
             raise capi.SphError(f"request {req.rid} does not fit the server's HBM budget")
         heapq.heappush(self.queue, (req.arrival, req.rid, req))
@@ -141,18 +176,26 @@ class Server:
         keep, finished = [], []
         for i, r in enumerate(self.resident):
             (finished if r.done >= r.req.steps else keep).append(i)
-        cons = self.pool.conserved() if finished else []
-        for i in finished:
-            r = self.resident[i]
-            self.results[r.req.rid] = Result(r.req.rid, self.pool.download(i), cons[i], r.done, r.req.arrival,
-                                             r.admitted, now)
+        t_r = time.perf_counter()
+        if self.response_fields:
+            from .stepper import download_async
+            for i in finished:
+                arrays, cons = download_async(self.pool, i, self.response_fields, self._ring)
+                self._pending.append((self.resident[i], arrays, cons, now))
+        else:
+            cons = self.pool.conserved() if finished else []
+            for i in finished:
+                r = self.resident[i]
+                self.results[r.req.rid] = Result(r.req.rid, self.pool.download(i), cons[i], r.done, r.req.arrival,
+                                                 r.admitted, now)
+        self.stats.retire_ms.append((time.perf_counter() - t_r) * 1e3)
         used_b = sum(self.run_bytes(self.resident[i].req) for i in keep)
-        used_p = sum(self.resident[i].req.ps.n for i in keep)
+        used_p = sum(self.resident[i].req.n for i in keep)
         used_c = sum(self.pool.cells_needed(self.resident[i].req.cfg) for i in keep)
         add: List[Request] = []
         while self.queue and self.queue[0][0] <= admit_until and len(keep) + len(add) < self.max_runs:
             req = self.queue[0][2]
-            b, n, c = self.run_bytes(req), req.ps.n, self.pool.cells_needed(req.cfg)
+            b, n, c = self.run_bytes(req), req.n, self.pool.cells_needed(req.cfg)
             if used_b + b > self.budget or used_p + n > self.max_particles or used_c + c > self.max_cells:
                 break  # FIFO: the head waits until enough retires
             heapq.heappop(self.queue)
@@ -160,12 +203,28 @@ class Server:
             used_b, used_p, used_c = used_b + b, used_p + n, used_c + c
         if not finished and not add:
             return
-        new_idx = self.pool.set_runs(keep, [(q.cfg, q.ps.n) for q in add])
+        t_m = time.perf_counter()
+        new_idx = self.pool.set_runs(keep, [(q.cfg, q.n) for q in add])  # synchronises the stream
+        self.stats.move_ms.append((time.perf_counter() - t_m) * 1e3)
+        t_f = time.perf_counter()
         # an admitted run's first batched step primes it (density, forces, first dt; no motion),
         # so it needs one step more than it asked for and no separate priming pass
         self.resident = [self.resident[i] for i in keep] + [_Resident(q, -1, now) for q in add]
         for j, q in zip(new_idx, add):
-            self.pool.upload(j, q.ps)
+            if q.ps is not None:
+                self.pool.upload(j, q.ps)
+            else:
+                self.pool.init_run(j, q.blocks, seed=q.seed)
+        self.stats.fill_ms.append((time.perf_counter() - t_f) * 1e3)
+
+    def _harvest(self):
+        """Turn completed asynchronous copy-outs into results (the stream has been synchronised)."""
+        for r, arrays, cons, fin in self._pending:
+            self.results[r.req.rid] = Result(r.req.rid, None, cons.as_dict(), r.done, r.req.arrival, r.admitted,
+                                             fin, fields={k: v.copy() for k, v in arrays.items()})
+        self._pending = []
+        if self._ring is not None:
+            self._ring.reset()
 
     def serve(self, requests: Optional[List[Request]] = None, max_steps: int = 1 << 30,
               realtime: bool = True) -> dict:
@@ -173,6 +232,11 @@ class Server:
         queue reaches it (arrival times only order the queue)."""
         for q in requests or []:
             self.submit(q)
+        # per-step device time from events recorded on the pool's stream (host work between steps,
+        # such as copying finished responses, does not inflate it)
+        profiled = hasattr(self.pool, "profile_begin")
+        if profiled:
+            self.pool.profile_begin(sum(q[2].steps + 1 for q in self.queue) + 8)
         self._t0 = time.perf_counter()
         n_steps = 0
         while (self.queue or self.resident) and n_steps < max_steps:
@@ -185,18 +249,28 @@ class Server:
                 continue
             t1 = time.perf_counter()
             self.pool.step(1)
+            self._harvest()  # responses completed by the last synchronisation: copied while the GPU steps
             self.pool.sync()
             self.stats.step_ms.append((time.perf_counter() - t1) * 1e3)
             self.stats.batch_runs.append(len(self.resident))
-            self.stats.batch_particles.append(sum(r.req.ps.n for r in self.resident))
+            self.stats.batch_particles.append(sum(r.req.n for r in self.resident))
             for r in self.resident:
                 r.done += 1
             n_steps += 1
         self._retire_and_admit(float("-inf"))  # retire the last runs, admit nothing
+        self.pool.sync()
+        self._harvest()
+        if profiled:
+            prof = self.pool.profile_end()
+            if len(prof["step_ms"]) == len(self.stats.step_ms):
+                self.stats.step_wall_ms = self.stats.step_ms
+                self.stats.step_ms = list(prof["step_ms"])
         return self.stats.summary()
 
     def close(self):
         self.pool.close()
+        if self._ring is not None:
+            self._ring.close()
 
 
 def poisson_arrivals(n: int, rate_per_s: float, seed: int = 0) -> np.ndarray:

@@ -57,6 +57,12 @@ class Context:
         s = ps.soa()
         check(self.lib.sph_upload(self.h, run, C.byref(s)), self.h, "sph_upload")
 
+    def init_run(self, run: int, blocks, seed: int = 0, jitter: float = 0.05):
+        """Fill a run on the device from lattice blocks (capi.LatticeBlock); see sph_init_run."""
+        arr = (capi.LatticeBlock * len(blocks))(*blocks)
+        check(self.lib.sph_init_run(self.h, run, arr, len(blocks), int(seed) & 0xFFFFFFFFFFFFFFFF, float(jitter)),
+              self.h, "sph_init_run")
+
     def download(self, run: int) -> ParticleSet:
         ps = ParticleSet(self.sizes[run])
         s = ps.soa()
@@ -156,6 +162,80 @@ class Context:
         t = Tuning()
         check(self.lib.sph_autotune(self.h, n_steps, C.byref(t)), self.h, "sph_autotune")
         return t.as_dict()
+
+
+# host columns of sph_host_soa by response field name: (attribute, dtype, components)
+RESPONSE_FIELDS = {
+    "id": ("id", np.uint64, 1), "mask": ("mask", np.uint8, 1), "species": ("species", np.uint8, 1),
+    "pos": ("pos", np.float64, 3), "vel": ("vel", np.float32, 3), "mass": ("mass", np.float32, 1),
+    "rho": ("density", np.float32, 1), "h": ("smoothing_length", np.float32, 1),
+    "u": ("internal_energy", np.float32, 1), "B": ("bfield", np.float32, 3), "psi": ("psi", np.float32, 1),
+    "dvdt": ("dvdt", np.float32, 3), "dudt": ("dudt", np.float32, 1), "dbdt": ("dbdt", np.float32, 3),
+    "dpsidt": ("dpsidt", np.float32, 1), "divb": ("divb", np.float32, 1), "vsig": ("vsig", np.float32, 1),
+    "nnb": ("n_neighbours", np.uint32, 1),
+}
+
+
+class PinnedRing:
+    """A block of pinned host memory handed out front to back; reset() once every copy into it has
+    completed. Buffers for sph_download_async."""
+
+    def __init__(self, nbytes: int):
+        self.lib = capi.load()
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        if self.lib.sph_host_alloc(self.nbytes, C.byref(p)) != 0:
+            raise capi.SphError(f"sph_host_alloc({self.nbytes}) failed")
+        self.base = p.value
+        self.off = 0
+
+    def take(self, dtype, shape) -> np.ndarray:
+        dt = np.dtype(dtype)
+        size = int(np.prod(shape)) * dt.itemsize
+        start = (self.off + 255) & ~255
+        if start + size > self.nbytes:
+            raise capi.SphError("pinned ring exhausted")
+        self.off = start + size
+        buf = (C.c_char * size).from_address(self.base + start)
+        return np.frombuffer(buf, dtype=dt).reshape(shape)
+
+    def reset(self):
+        self.off = 0
+
+    def close(self):
+        if getattr(self, "base", None):
+            self.lib.sph_host_free(C.c_void_p(self.base))
+            self.base = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def download_async(ctx: "Context", run: int, fields, ring: PinnedRing):
+    """Enqueue the copy-out of `fields` (names of RESPONSE_FIELDS) and the run's conserved record into
+    `ring`; returns (arrays, conserved): valid once the context's stream has been synchronised."""
+    from .capi import HostSoA
+    n = ctx.sizes[run]
+    s = HostSoA()
+    s.n = n
+    out = {}
+    for f in fields:
+        attr, dt, comp = RESPONSE_FIELDS[f]
+        arr = ring.take(dt, (comp, n) if comp > 1 else (n,))
+        out[f] = arr
+        ptype = C.POINTER(np.ctypeslib.as_ctypes_type(dt))
+        if comp > 1:
+            for a in range(comp):
+                getattr(s, attr)[a] = arr[a].ctypes.data_as(ptype)
+        else:
+            setattr(s, attr, arr.ctypes.data_as(ptype))
+    cons_buf = ring.take(np.float64, (8,))
+    cons = Conserved.from_address(cons_buf.ctypes.data)
+    check(ctx.lib.sph_download_async(ctx.h, run, C.byref(s), C.byref(cons)), ctx.h, "sph_download_async")
+    return out, cons
 
 
 class Pool(Context):

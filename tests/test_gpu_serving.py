@@ -8,7 +8,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from spearhed_amd import ic
+from spearhed_amd import ic, setups
 from spearhed_amd.serving import Request, Server, run_bytes
 from spearhed_amd.stepper import Context, Pool
 
@@ -17,9 +17,13 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("pos", "vel", "u", "rho", "B", "psi", "dvdt", "dbdt", "nnb")
 
 
-def _alone(cfg, ps, steps):
-    with Context([cfg], [ps.n], precision="mixed") as ctx:
-        ctx.upload(0, ps)
+def _alone(cfg, ps, steps, blocks=None, seed=0):
+    n = ps.n if ps is not None else Request(0, cfg, None, 0, blocks=blocks).n
+    with Context([cfg], [n], precision="mixed") as ctx:
+        if ps is not None:
+            ctx.upload(0, ps)
+        else:
+            ctx.init_run(0, blocks, seed=seed)
         ctx.prime()
         if steps:
             ctx.step(steps)
@@ -74,9 +78,13 @@ def test_pool_rejects_over_capacity():
 def test_served_results_equal_alone():
     probs = [ic.kelvin_helmholtz(2500, 40), ic.brio_wu(2000, 41), ic.rotor(3000, 42),
              ic.kelvin_helmholtz(5000, 43), ic.orszag_tang_n(2500, 44)]
-    steps = [3, 5, 2, 4, 3]
+    steps = [3, 5, 2, 4, 3, 4, 2]
     reqs = [Request(r, c, p, steps[r], arrival=float(r)) for r, (c, p) in enumerate(probs)]
-    budget = 3 * max(run_bytes(q.cfg, q.ps.n) for q in reqs)  # forces retire/admit cycles
+    # two requests filled on the device from lattice blocks
+    for r, make in ((5, setups.rotor), (6, setups.brio_wu)):
+        cfg, blocks, _ = make(3000)
+        reqs.append(Request(r, cfg, None, steps[r], arrival=float(r), blocks=blocks, seed=100 + r))
+    budget = 3 * max(run_bytes(q.cfg, q.n) for q in reqs)  # forces retire/admit cycles
     srv = Server(budget, precision="mixed", max_runs=8)
     try:
         summary = srv.serve(reqs, realtime=False)
@@ -85,7 +93,7 @@ def test_served_results_equal_alone():
     print("serve:", summary, flush=True)
     assert sorted(srv.results) == list(range(len(reqs)))
     for q in reqs:
-        a, ca = _alone(q.cfg, q.ps, q.steps)
+        a, ca = _alone(q.cfg, q.ps, q.steps, q.blocks, q.seed)
         res = srv.results[q.rid]
         _same(a, res.ps.sorted_by_id())
         assert res.conserved["time"] == ca["time"]
@@ -156,3 +164,28 @@ def test_first_step_after_upload_primes(dt_fixed):
     (a, ca), (b, cb) = outs
     _same(a, b)
     assert ca == cb
+
+
+def test_served_response_fields_equal_alone():
+    """The asynchronous copy-out path (pinned ring, sph_download_async) returns the same selected
+    fields and conserved record as the run stepped alone."""
+    probs = [ic.kelvin_helmholtz(3000, 60), ic.orszag_tang_n(2500, 61), ic.rotor(2600, 62), ic.brio_wu(2400, 63)]
+    steps = [3, 2, 4, 3]
+    reqs = [Request(r, c, p, steps[r], arrival=float(r)) for r, (c, p) in enumerate(probs)]
+    fields = ("id", "pos", "vel", "rho", "u", "B", "psi", "nnb")
+    budget = 2 * max(run_bytes(q.cfg, q.n) for q in reqs)
+    srv = Server(budget, precision="mixed", max_runs=8, response_fields=fields)
+    try:
+        srv.serve(reqs, realtime=False)
+    finally:
+        srv.close()
+    for q in reqs:
+        a, ca = _alone(q.cfg, q.ps, q.steps)
+        got = srv.results[q.rid]
+        assert got.ps is None
+        order = np.argsort(got.fields["id"])
+        assert np.array_equal(got.fields["id"][order], a.id)
+        for f in ("pos", "vel", "rho", "u", "B", "psi", "nnb"):
+            assert np.array_equal(got.fields[f][..., order], getattr(a, f)), f
+        for k in ("mass", "energy", "time", "dt_next"):
+            assert got.conserved[k] == ca[k], k
