@@ -63,7 +63,9 @@ extern "C" {
 typedef struct sph_run_config {
     double box_lo[3];     /* lower corner of the box */
     double box_len[3];    /* edge lengths; periodic axes wrap with this period */
-    int32_t periodic[3];  /* 1 = periodic axis (minimum image through the supercell stencil) */
+    int32_t periodic[3];  /* 1 = periodic axis (minimum image through the supercell stencil); on a
+                             non-periodic axis a position outside the box is binned to the edge
+                             supercell (so walls may sit just outside) */
     int32_t kernel;       /* SPH_KERNEL_* */
     float h0;             /* smoothing length that sets the interaction radius H = support * h0
                              (reference: sph.param h0, Simulation.x.cpp:263-264) */
@@ -112,7 +114,9 @@ typedef struct sph_conserved {
     double mass;
     double momentum[3];
     double energy;   /* sum m (v^2/2 + u + B^2/(2 rho)) */
-    double dt_next;  /* CFL dt for the next step */
+    double dt_next;  /* CFL dt for the next step: the minimum over live targets; +inf when the run has
+                        none (only walls, tracers or dead particles), and a run whose dt is +inf does
+                        not advance (no drift, kick or time) until dt_fixed or a host dt is given */
     double time;     /* simulated time of the run */
     double c_h;      /* cleaning speed used in the last force pass */
 } sph_conserved;

@@ -68,17 +68,16 @@ int orc_bin(const sph_run_config* cfg, int32_t prec, int64_t n, double* const po
     for (int64_t i = 0; i < n; ++i) {
         int c[3];
         for (int a = 0; a < 3; ++a) {
+            /* clamped before the conversion: outside the box (or not finite) maps to an edge cell */
             int ci;
             if (prec == ORC_PREC_PLAIN) {
                 float x = (float)pos[a][i];
                 float t = (x - g.lo32[a]) * g.inv_s32[a];
-                ci = (int)floorf(t);
+                ci = (int)fminf(fmaxf(floorf(t), 0.0f), (float)(g.n[a] - 1));
             } else {
                 double t = (pos[a][i] - g.lo[a]) * g.inv_s[a];
-                ci = (int)floor(t);
+                ci = (int)fmin(fmax(floor(t), 0.0), (double)(g.n[a] - 1));
             }
-            if (ci < 0) ci = 0;
-            if (ci > g.n[a] - 1) ci = g.n[a] - 1;
             c[a] = ci;
         }
         key[i] = (uint32_t)(c[0] + g.n[0] * (c[1] + g.n[1] * c[2]));

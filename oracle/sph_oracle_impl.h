@@ -151,9 +151,7 @@ static void FN(pair_coords)(const orc_grid* g, int32_t prec, const orc_state* st
             double x = st->pos[a][i];
             if (prec == ORC_PREC_MIXED) {
                 double t = (x - g->lo[a]) * g->inv_s[a];
-                int c = (int)floor(t);
-                if (c < 0) c = 0;
-                if (c > g->n[a] - 1) c = g->n[a] - 1;
+                int c = (int)fmin(fmax(floor(t), 0.0), (double)(g->n[a] - 1));
                 pc[a][i] = (REAL)((x - g->lo[a]) - (double)c * g->s[a]);
             } else {
                 pc[a][i] = (REAL)x;
@@ -703,11 +701,14 @@ int FN(orc_step)(const sph_run_config* cfg, int32_t prec, orc_state* st, int32_t
     for (int s = 0; s < n_steps && !rc; ++s) {
         double dt = cfg->dt_fixed > 0 ? cfg->dt_fixed : st->dt;
         if (!(dt > 0)) { rc = -4; break; }
-        rc = FN(orc_drift)(cfg, prec, st, dt);
+        /* dt = +inf: no live target bounds the CFL step, so the run does not advance (no drift, no
+         * kick, no time); its derivatives and sums are still evaluated */
+        const int advance = dt < HUGE_VAL;
+        if (advance) rc = FN(orc_drift)(cfg, prec, st, dt);
         if (!rc) rc = FN(orc_resort)(cfg, prec, st, cs);
         if (!rc) rc = FN(orc_density)(cfg, prec, st, cs);
         if (!rc) rc = FN(orc_forces)(cfg, prec, st, cs);
-        if (!rc) rc = FN(orc_kick)(cfg, prec, st, dt, &c);
+        if (!rc) rc = advance ? FN(orc_kick)(cfg, prec, st, dt, &c) : FN(orc_reduce)(cfg, prec, st, &c);
         if (!rc) st->dt = c.dt_next;
     }
     if (out) *out = c;

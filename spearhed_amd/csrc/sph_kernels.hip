@@ -93,8 +93,9 @@ __global__ __launch_bounds__(256) void k_drift_bin(const RunDev* __restrict__ ru
     if (active) {
         const double dt = dt_host > 0.0 ? dt_host : W.dt[r];
         const uint8_t live = S.mask[i];
-        // a run whose next step primes it (fresh initial data) does not move in that step
-        const bool mov = do_drift && is_movable(S.species[i], live) && !W.prime_flag[r];
+        // a run whose next step primes it (fresh initial data) does not move in that step, nor does
+        // a run without a CFL bound (no live target: dt = +inf)
+        const bool mov = do_drift && is_movable(S.species[i], live) && !W.prime_flag[r] && isfinite(dt);
         int c[3];
         for (int a = 0; a < 3; ++a) {
 #pragma clang fp contract(off)
@@ -110,7 +111,7 @@ __global__ __launch_bounds__(256) void k_drift_bin(const RunDev* __restrict__ ru
                     }
                     S.pos64[a][i] = x;
                 }
-                ci = (int)floor((x - R.lo[a]) * R.inv_s[a]);
+                ci = (int)fmin(fmax(floor((x - R.lo[a]) * R.inv_s[a]), 0.0), (double)(R.n[a] - 1));
             } else {
                 float x = S.pos32[a][i];
                 if (mov) {
@@ -122,9 +123,9 @@ __global__ __launch_bounds__(256) void k_drift_bin(const RunDev* __restrict__ ru
                     }
                     S.pos32[a][i] = x;
                 }
-                ci = (int)floorf((x - R.lo32[a]) * R.inv_s32[a]);
+                ci = (int)fminf(fmaxf(floorf((x - R.lo32[a]) * R.inv_s32[a]), 0.0f), (float)(R.n[a] - 1));
             }
-            ci = ci < 0 ? 0 : (ci > R.n[a] - 1 ? R.n[a] - 1 : ci);
+            // clamped before the conversion: outside the box (or not finite) maps to an edge cell
             c[a] = ci;
         }
         key = (uint32_t)(R.c0 + c[0] + R.n[0] * (c[1] + R.n[1] * c[2]));
@@ -1184,7 +1185,7 @@ __global__ __launch_bounds__(256) void k_kick(const RunDev* __restrict__ runs, S
             float u = S.u[i];
             const float ax = W.dvdt[0][i], ay = W.dvdt[1][i], az = W.dvdt[2][i];
             float bx = 0.f, by = 0.f, bz = 0.f;
-            const bool kick = do_kick && !W.prime_flag[r];
+            const bool kick = do_kick && !W.prime_flag[r] && isfinite(dt);  // no CFL bound: no kick
             if (kick) {
                 vx = vx + ax * dtf;
                 vy = vy + ay * dtf;
@@ -1286,7 +1287,7 @@ __global__ __launch_bounds__(256) void k_finalize(const RunDev* __restrict__ run
         const double dt_used = dt_host > 0.0 ? dt_host : W.dt[r];
         const int priming = W.prime_flag[r];
         double t = W.time[r];
-        if (advance && !priming) t += dt_used;
+        if (advance && !priming && isfinite(dt_used)) t += dt_used;
         W.time[r] = t;
         const double dtn = R.dt_fixed > 0.0 ? R.dt_fixed : red[0][0];
         double* c = W.cons + (int64_t)r * 8;
