@@ -449,8 +449,12 @@ int FN(orc_forces)(const sph_run_config* cfg, int32_t prec, orc_state* st, const
                     REAL fiso = P_i.pr * ms_i + P_j.pr * ms_j;
                     REAL cax = -fiso * d[0], cay = -fiso * d[1], caz = -fiso * d[2];
                     DV += ms_i * vr;
-                    sdvv += fabs((double)(ms_i * vr));
-                    sdu += fabs((double)(P_i.ph * ms_i * vr));
+                    /* scales use the magnitudes before cancellation: sum_a |dv_a d_a| for v.r (and
+                     * sum_a |B_a d_a| for B.r below), the bound of an FP32 evaluation in any order */
+                    const double vdn = fabs((double)dvx * (double)d[0]) + fabs((double)dvy * (double)d[1])
+                                       + fabs((double)dvz * (double)d[2]);
+                    sdvv += fabs((double)ms_i) * vdn;
+                    sdu += fabs((double)(P_i.ph * ms_i)) * vdn;
                     /* signal velocity */
                     REAL w = vr * invR;
                     REAL vs = P_i.c + P_j.c - beta * (w < (REAL)0 ? w : (REAL)0);
@@ -467,7 +471,7 @@ int FN(orc_forces)(const sph_run_config* cfg, int32_t prec, orc_state* st, const
                         cay -= gv * d[1];
                         caz -= gv * d[2];
                         AVU += gv * vr;
-                        sdu += 0.5 * fabs((double)(gv * vr));
+                        sdu += fabs((double)gv) * vdn;
                     }
                     ax += cax;
                     ay += cay;
@@ -491,18 +495,23 @@ int FN(orc_forces)(const sph_run_config* cfg, int32_t prec, orc_state* st, const
                         DB += db;
                         double bn_i = sqrt((double)bi[0] * (double)bi[0] + (double)bi[1] * (double)bi[1] + (double)bi[2] * (double)bi[2]);
                         double bn_j = sqrt((double)Bx[j] * (double)Bx[j] + (double)By[j] * (double)By[j] + (double)Bz[j] * (double)Bz[j]);
-                        cnorm += fabs((double)ti) * bn_i + fabs((double)tj) * bn_j
-                                 + (double)borve * fabs((double)ti + (double)tj) * bn_i;
+                        const double bdi = fabs((double)bi[0] * (double)d[0]) + fabs((double)bi[1] * (double)d[1])
+                                           + fabs((double)bi[2] * (double)d[2]);
+                        const double bdj = fabs((double)Bx[j] * (double)d[0]) + fabs((double)By[j] * (double)d[1])
+                                           + fabs((double)Bz[j] * (double)d[2]);
+                        const double ati = fabs((double)(P_i.bdr * ms_i)) * bdi;
+                        const double atj = fabs((double)(P_j.bdr * ms_j)) * bdj;
+                        cnorm += ati * bn_i + atj * bn_j + (double)borve * (ati + atj) * bn_i;
                         double dvn = sqrt((double)dvx * (double)dvx + (double)dvy * (double)dvy + (double)dvz * (double)dvz);
-                        sdb += (fabs((double)wv) * dvn + bn_i * fabs((double)(ms_i * vr))) / (double)rho[i];
-                        sdiv += fabs((double)db) / (double)rho[i];
+                        sdb += (fabs((double)ms_i) * bdi * dvn + bn_i * fabs((double)ms_i) * vdn) / (double)rho[i];
+                        sdiv += fabs((double)ms_i) * (bdi + bdj) / (double)rho[i];
                         if (clean) {
                             REAL fps = P_i.psr * ms_i + P_j.psr * ms_j;
                             PSx += fps * d[0];
                             PSy += fps * d[1];
                             PSz += fps * d[2];
                             double dn = sqrt((double)d[0] * (double)d[0] + (double)d[1] * (double)d[1] + (double)d[2] * (double)d[2]);
-                            sdb += (double)rho[i] * fabs((double)fps) * dn;
+                            sdb += (double)rho[i] * (fabs((double)(P_i.psr * ms_i)) + fabs((double)(P_j.psr * ms_j))) * dn;
                         }
                     }
                     sa += cnorm;

@@ -8,8 +8,9 @@ from spearhed_amd.particles import ParticleSet
 from spearhed_amd.stepper import Context
 
 # Per-particle tolerance: |gpu - oracle| <= RTOL * S, where S is the sum over pairs of |term| of the
-# accumulated field (the quantity an FP32 summation error is bounded by). Density terms are all
-# positive, so for rho this is a plain relative error.
+# accumulated field, each dot product inside a term (v.r, B.r) taken at its magnitude before
+# cancellation, sum_a |v_a r_a| (the quantity an FP32 evaluation error is bounded by). Density terms
+# are all positive, so for rho this is a plain relative error.
 RTOL = 1e-5
 # Global FP64 sums (mass, momentum, energy): relative to the sum of magnitudes.
 RTOL_GLOBAL = 1e-6
