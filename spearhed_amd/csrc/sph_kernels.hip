@@ -246,8 +246,13 @@ __global__ __launch_bounds__(256) void k_gather(const RunDev* __restrict__ runs,
     const uint32_t c = W.key[i];
     const uint32_t cb = W.cs[c], ce = W.cs[c + 1];
     const uint64_t myid = W.tmpid[q];
-    uint32_t rank = 0;
-    for (uint32_t m = cb; m < ce; ++m) rank += (W.tmpid[m] < myid) ? 1u : 0u;
+    // rank within the supercell = number of smaller ids in it; four independent loads per round
+    uint32_t rank = 0, mm = cb;
+    for (; mm + 4 <= ce; mm += 4) {
+        const uint64_t a0 = W.tmpid[mm], a1 = W.tmpid[mm + 1], a2 = W.tmpid[mm + 2], a3 = W.tmpid[mm + 3];
+        rank += (a0 < myid ? 1u : 0u) + (a1 < myid ? 1u : 0u) + (a2 < myid ? 1u : 0u) + (a3 < myid ? 1u : 0u);
+    }
+    for (; mm < ce; ++mm) rank += (W.tmpid[mm] < myid) ? 1u : 0u;
     const uint32_t f = cb + rank;
     if (rank == 0) W.count[c] = 0u;  // histogram reset for the next resort
 
@@ -853,7 +858,7 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
                     sp[1][e] = W.P1[j];
                     sp[2][e] = W.P2[j];
                     sp[3][e] = W.P3[j];
-                    if (MHD) sp[4][e] = W.P4[j];
+                    if constexpr (MHD) sp[4][e] = W.P4[j];
                 } else {
                     sp[0][e] = make_float4(kSentinel, kSentinel, kSentinel, 0.0f);
                 }
@@ -880,7 +885,7 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
                         msk &= msk - 1;
                         const int e = k + (sb + ii) * K;
                         float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if (MHD) q4 = sp[4][e];
+                        if constexpr (MHD) q4 = sp[4][e];
                         force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
                                         beta, mhd, clean, A);
                     }
@@ -890,7 +895,7 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
                     for (int q = k; q < gtot; q += K) {
                         const int e = lst[lm.gfirst * 32 + q];
                         float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if (MHD) q4 = sp[4][e];
+                        if constexpr (MHD) q4 = sp[4][e];
                         force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
                                         beta, mhd, clean, A);
                     }
@@ -1113,7 +1118,7 @@ __global__ __launch_bounds__(64) void k_forces_cell(const RunDev* __restrict__ r
                 sp[1][e] = W.P1[j];
                 sp[2][e] = W.P2[j];
                 sp[3][e] = W.P3[j];
-                if (MHD) sp[4][e] = W.P4[j];
+                if constexpr (MHD) sp[4][e] = W.P4[j];
             }
             __syncthreads();
             if (listed) {
@@ -1128,7 +1133,7 @@ __global__ __launch_bounds__(64) void k_forces_cell(const RunDev* __restrict__ r
                 for (int q = k; q < g; q += K) {
                     const int e = (int)lst[lo + q] - base;
                     float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (MHD) q4 = sp[4][e];
+                    if constexpr (MHD) q4 = sp[4][e];
                     force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
                                     beta, mhd, clean, A);
                 }
@@ -1151,7 +1156,7 @@ __global__ __launch_bounds__(64) void k_forces_cell(const RunDev* __restrict__ r
                         msk &= msk - 1;
                         const int e = cc - base + ii;
                         float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if (MHD) q4 = sp[4][e];
+                        if constexpr (MHD) q4 = sp[4][e];
                         force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
                                         beta, mhd, clean, A);
                     }
