@@ -66,11 +66,20 @@ static inline CountF floor(CountF a) { return CountF(std::floor(a.v)); }
 using std::floor;
 using std::sqrt;
 
+// a fused multiply-add counts as its multiply and its add
+static inline CountF count_fma(CountF a, CountF b, CountF c)
+{
+    g_ops += 2;
+    return CountF(std::fma(a.v, b.v, c.v));
+}
+
 #define REAL CountF
 #define S cnt
+#define RFMA count_fma
 #include "sph_oracle_impl.h"
 #undef REAL
 #undef S
+#undef RFMA
 
 extern "C" {
 // One density (or force) pass on an FP32-layout state; *ops receives the operations counted.

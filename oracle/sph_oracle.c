@@ -118,12 +118,16 @@ int orc_sort(int64_t n, const uint32_t* key, const uint64_t* id, int32_t ncell, 
 
 #define REAL float
 #define S f32
+#define RFMA fmaf
 #include "sph_oracle_impl.h"
 #undef REAL
 #undef S
+#undef RFMA
 
 #define REAL double
 #define S f64
+#define RFMA fma
 #include "sph_oracle_impl.h"
 #undef REAL
 #undef S
+#undef RFMA

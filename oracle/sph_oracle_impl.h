@@ -188,18 +188,16 @@ static inline int FN(nb_cell)(const orc_grid* g, int32_t prec, const sph_run_con
     return 1;
 }
 
-/* Squared distance of own pair coordinate to a staged neighbour, evaluated as
- * ((dx*dx + dy*dy) + dz*dz) without contraction (ParticleParticleInteraction.hpp:281-289). */
+/* Squared distance of own pair coordinate to a staged neighbour: dx^2 + dy^2 + dz^2
+ * (ParticleParticleInteraction.hpp:281-289), evaluated as fma(dz, dz, fma(dy, dy, dx*dx)), the
+ * GPU kernels' evaluation (the reference leaves the contraction to its compiler). */
 static inline REAL FN(geom)(const REAL pi[3], REAL* const pc[3], int64_t j, const REAL shift[3], REAL d[3])
 {
     for (int a = 0; a < 3; ++a) {
         REAL staged = pc[a][j] + shift[a];
         d[a] = pi[a] - staged;
     }
-    REAL r2 = d[0] * d[0];
-    r2 = r2 + d[1] * d[1];
-    r2 = r2 + d[2] * d[2];
-    return r2;
+    return RFMA(d[2], d[2], RFMA(d[1], d[1], d[0] * d[0]));
 }
 
 int FN(orc_permute)(orc_state* st, const int64_t* perm)

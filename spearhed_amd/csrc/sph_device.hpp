@@ -146,19 +146,17 @@ __device__ __forceinline__ float kgrad(int kern, float rr, float invR, float q, 
 __device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// Squared distance from the own pair coordinate to a staged (pre-shifted) neighbour, rounded
-// exactly as ((dx*dx + dy*dy) + dz*dz) with no fused multiply-add: the cull decision and the
-// neighbour counts are part of the bit-exact contract.
+// Squared distance from the own pair coordinate to a staged (pre-shifted) neighbour, as
+// fma(dz, dz, fma(dy, dy, dx*dx)) with every other contraction off: the cull decision and the
+// neighbour counts are part of the bit-exact contract, and correctly rounded fused operations give
+// the same bits on any IEEE machine.
 __device__ __forceinline__ float geom_r2(float xi, float yi, float zi, float4 p, float& dx, float& dy, float& dz)
 {
 #pragma clang fp contract(off)
     dx = xi - p.x;
     dy = yi - p.y;
     dz = zi - p.z;
-    float r2 = dx * dx;
-    r2 = r2 + dy * dy;
-    r2 = r2 + dz * dz;
-    return r2;
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));  // three roundings, reproduced exactly by the oracle
 }
 
 }  // namespace sph
