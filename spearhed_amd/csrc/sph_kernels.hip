@@ -374,14 +374,14 @@ __device__ __forceinline__ uint32_t cull32(const float4* __restrict__ sp, int e0
         for (int ii = 0; ii < 32; ++ii) {
             float dx, dy, dz;
             const float r2 = geom_r2(xi, yi, zi, sp[e0 + ii * K], dx, dy, dz);
-            m = (m << 1) | (__float_as_uint(r2 - H2) >> 31);
+            m = __builtin_amdgcn_alignbit(m, __float_as_uint(r2 - H2), 31);  // (m << 1) | sign
         }
         return __builtin_bitreverse32(m);
     }
     for (int ii = 0; ii < lim; ++ii) {
         float dx, dy, dz;
         const float r2 = geom_r2(xi, yi, zi, sp[e0 + ii * K], dx, dy, dz);
-        m = (m << 1) | (__float_as_uint(r2 - H2) >> 31);
+        m = __builtin_amdgcn_alignbit(m, __float_as_uint(r2 - H2), 31);  // (m << 1) | sign
     }
     return __builtin_bitreverse32(m) >> (32 - lim);
 }
