@@ -1132,76 +1132,82 @@ int sph_profile_end(sph_ctx* c, sph_profile* out, double* step_ms)
 int sph_autotune(sph_ctx* c, int32_t n_steps, sph_tuning* best)
 {
     if (!c || n_steps < 1) return SPH_E_INVALID;
-    // snapshot the current buffer set and run scalars so timing leaves the state untouched
+    // device scratch of the search, released on every path
+    struct Scratch {
+        void* backup = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Scratch()
+        {
+            if (backup) hipFree(backup);
+            if (e0) hipEventDestroy(e0);
+            if (e1) hipEventDestroy(e1);
+        }
+    } sc;
+    // snapshot the whole arena and the run configurations: timing leaves the state untouched
     HIPTRY(c, hipStreamSynchronize(c->stream));
     const int cur0 = c->cur;
-    const bool b0 = c->binned, s0 = c->sorted, d0 = c->dens_valid;
-    const size_t half = c->arena_bytes;  // whole arena (simple and correct)
-    void* backup = nullptr;
-    HIPTRY(c, hipMalloc(&backup, half));
-    HIPTRY(c, hipMemcpyAsync(backup, c->arena, half, hipMemcpyDeviceToDevice, c->stream));
-    std::vector<sph_run_config> cfg0 = c->cfg;
+    const size_t bytes = c->arena_bytes;
+    HIPTRY(c, hipMalloc(&sc.backup, bytes));
+    HIPTRY(c, hipEventCreate(&sc.e0));
+    HIPTRY(c, hipEventCreate(&sc.e1));
+    HIPTRY(c, hipMemcpyAsync(sc.backup, c->arena, bytes, hipMemcpyDeviceToDevice, c->stream));
+    const std::vector<sph_run_config> cfg0 = c->cfg;
     const sph_tuning t0 = c->tune;
-    double best_t = 1e300;
-    sph_tuning best_tune = t0;
-    float best_cf = -1.0f;
-    hipEvent_t e0, e1;
-    HIPTRY(c, hipEventCreate(&e0));
-    HIPTRY(c, hipEventCreate(&e1));
-    // candidate launch shapes: chunked passes (pair_balance 0/1) and whole-cell passes (2)
+
+    // candidate launch shapes: chunked passes (pair_balance 0/1) and whole-cell passes (2), each at
+    // supercell edges of 1 and 1.15 interaction radii
     std::vector<sph_tuning> cands;
     for (int dc : {128, 256})
         for (int fc : {64, 128})
             for (int k : {16, 64})
                 for (int bal : {0, 1}) cands.push_back({dc, fc, t0.waves_per_cu, k, bal});
     for (int k : {16, 64}) cands.push_back({512, 128, t0.waves_per_cu, k, 2});
-
     const float cfs[2] = {1.0f, 1.15f};
-    int rc = SPH_OK;
-    for (float cf : cfs) {
-        for (int r = 0; r < c->nruns && !rc; ++r) c->cfg[r].cell_factor = std::max(cf, cfg0[r].cell_factor);
-        for (const sph_tuning& cand : cands) {
-                    if (rc) break;
-                    HIPTRY(c, hipMemcpyAsync(c->arena, backup, half, hipMemcpyDeviceToDevice, c->stream));
-                    c->cur = cur0;
-                    rc = regrid(c);
-                    if (rc) break;
-                    c->tune = cand;
-                    rc = do_step(c);  // warm-up
-                    if (rc) break;
-                    HIPTRY(c, hipEventRecord(e0, c->stream));
-                    for (int s = 0; s < n_steps && !rc; ++s) rc = do_step(c);
-                    HIPTRY(c, hipEventRecord(e1, c->stream));
-                    HIPTRY(c, hipEventSynchronize(e1));
-                    float ms = 0;
-                    HIPTRY(c, hipEventElapsedTime(&ms, e0, e1));
-                    if (ms < best_t) {
-                        best_t = ms;
-                        best_tune = c->tune;
-                        best_cf = cf;
-                    }
+
+    double best_t = 1e300;
+    sph_tuning best_tune = t0;
+    float best_cf = -1.0f;
+    auto search = [&]() -> int {
+        for (float cf : cfs) {
+            for (int r = 0; r < c->nruns; ++r) c->cfg[r].cell_factor = std::max(cf, cfg0[r].cell_factor);
+            for (const sph_tuning& cand : cands) {
+                HIPTRY(c, hipMemcpyAsync(c->arena, sc.backup, bytes, hipMemcpyDeviceToDevice, c->stream));
+                c->cur = cur0;
+                int rc = regrid(c);
+                if (rc) return rc;
+                c->tune = cand;
+                if ((rc = do_step(c))) return rc;  // warm-up
+                HIPTRY(c, hipEventRecord(sc.e0, c->stream));
+                for (int s = 0; s < n_steps; ++s)
+                    if ((rc = do_step(c))) return rc;
+                HIPTRY(c, hipEventRecord(sc.e1, c->stream));
+                HIPTRY(c, hipEventSynchronize(sc.e1));
+                float ms = 0;
+                HIPTRY(c, hipEventElapsedTime(&ms, sc.e0, sc.e1));
+                if (ms < best_t) {
+                    best_t = ms;
+                    best_tune = cand;
+                    best_cf = cf;
                 }
-    }
-    // restore the state, apply the winner
-    HIPTRY(c, hipMemcpyAsync(c->arena, backup, half, hipMemcpyDeviceToDevice, c->stream));
-    HIPTRY(c, hipStreamSynchronize(c->stream));
-    hipFree(backup);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    for (int r = 0; r < c->nruns; ++r) {
-        c->cfg[r] = cfg0[r];
-        if (best_cf > 0) c->cfg[r].cell_factor = std::max(best_cf, cfg0[r].cell_factor);
-    }
+            }
+        }
+        return SPH_OK;
+    };
+    const int rc = search();
+
+    // restore the state; apply the winner only if the search completed
+    c->cfg = cfg0;
+    if (!rc && best_cf > 0)
+        for (int r = 0; r < c->nruns; ++r) c->cfg[r].cell_factor = std::max(best_cf, cfg0[r].cell_factor);
+    c->tune = rc ? t0 : best_tune;
     c->cur = cur0;
-    int rc2 = regrid(c);
+    HIPTRY(c, hipMemcpyAsync(c->arena, sc.backup, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    const int rc2 = regrid(c);
     c->binned = false;  // counts were cleared by regrid
     c->sorted = false;
     c->dens_valid = false;
-    (void)b0;
-    (void)s0;
-    (void)d0;
-    c->tune = best_tune;
-    if (best) *best = best_tune;
+    if (best) *best = c->tune;
     return rc ? rc : rc2;
 }
 

@@ -10,6 +10,7 @@
 * Tracers under a fixed dt drift by exactly v dt, as the oracle.
 * Positions outside a non-periodic box bin to the edge supercells: keys, permutation and cell
   starts bit-exact against the oracle.
+* The autotuner's search leaves the context's state as it found it.
 """
 from __future__ import annotations
 
@@ -154,3 +155,23 @@ def test_sort_with_positions_outside_a_nonperiodic_box(prec):
     assert np.array_equal(cs, ocs), "cell starts differ"
     assert np.array_equal(g.id, ps.id[perm]), "sort permutation differs"
     assert np.array_equal(key, okey[perm]), "supercell keys differ"
+
+
+def test_autotune_leaves_the_state_untouched():
+    """sph_autotune times candidate launch shapes on the context's own data and restores it: the
+    state, the run's clock and dt after the search are those before it, and the winner is applied."""
+    cfg, ps = ic.mhd_turbulence(n=12, seed=29)
+    with Context([cfg], [ps.n], precision="mixed") as ctx:
+        ctx.upload(0, ps)
+        ctx.step(3)
+        ctx.sync()
+        before = ctx.download(0).sorted_by_id()
+        c_before = ctx.conserved()[0]
+        tuned = ctx.autotune(1)
+        after = ctx.download(0).sorted_by_id()
+        c_after = ctx.conserved()[0]
+        assert ctx.tuning == tuned
+    for f in ("id", "pos", "vel", "rho", "u", "B", "psi", "h", "mass"):
+        assert np.array_equal(getattr(before, f), getattr(after, f)), f
+    for k in ("time", "dt_next", "mass", "energy"):
+        assert c_before[k] == c_after[k], k
