@@ -317,8 +317,8 @@ int do_step(sph_ctx* c)
 int reserve(sph_ctx* c, int64_t cap_p, int64_t cap_c, int cap_r, int cap_kb)
 {
     if (cap_p < 1 || cap_r < 1 || cap_c < 1) return fail(c, SPH_E_INVALID, "capacities must be >= 1");
-    if (cap_p >= (1LL << 32) - 1) return fail(c, SPH_E_CAPACITY, "more than 2^32-1 particles in one context");
-    if (cap_p >= (1LL << 27)) return fail(c, SPH_E_CAPACITY, "more than 2^27 particles in one context");
+    // particle indices and the cell-count scan are 32-bit signed on the device
+    if (cap_p > (1LL << 30)) return fail(c, SPH_E_CAPACITY, "more than 2^30 particles in one context");
     if (cap_c >= (1LL << 30)) return fail(c, SPH_E_CAPACITY, "too many supercells");
     c->cap_particles = cap_p;
     c->ncell_cap = (int)cap_c;
@@ -339,7 +339,7 @@ int reserve(sph_ctx* c, int64_t cap_p, int64_t cap_c, int cap_r, int cap_kb)
 }
 
 // Lay runs out contiguously (particles, supercells, kick blocks) and upload the run and block
-// tables. Runs flagged in `fresh` get their dt reset (dt_fixed or 0) and are marked for priming.
+// tables with the given per-run dt and time; runs flagged in `fresh` are marked for priming.
 int layout(sph_ctx* c, const std::vector<sph_run_config>& cfg, const std::vector<int64_t>& sizes,
            const std::vector<char>& fresh, const std::vector<double>& dt, const std::vector<double>& time)
 {
@@ -396,21 +396,23 @@ int layout(sph_ctx* c, const std::vector<sph_run_config>& cfg, const std::vector
 // Re-derive every run's grid (after a cell_factor change) keeping particle offsets.
 int regrid(sph_ctx* c)
 {
-    int c0 = 0;
+    std::vector<RunDev> runs(c->runs_h);  // committed only if every run fits
+    int64_t c0 = 0;
     for (int r = 0; r < c->nruns; ++r) {
         RunDev R;
         std::string why;
         if (!make_run(c->cfg[r], R, why)) return fail(c, SPH_E_INVALID, why);
-        R.p0 = c->runs_h[r].p0;
-        R.np = c->runs_h[r].np;
-        R.kb_first = c->runs_h[r].kb_first;
-        R.kb_count = c->runs_h[r].kb_count;
-        R.c0 = c0;
+        R.p0 = runs[r].p0;
+        R.np = runs[r].np;
+        R.kb_first = runs[r].kb_first;
+        R.kb_count = runs[r].kb_count;
+        R.c0 = (int32_t)c0;
         c0 += R.ncell;
-        c->runs_h[r] = R;
+        runs[r] = R;
+        if (c0 > c->ncell_cap) return fail(c, SPH_E_CAPACITY, "supercell table capacity exceeded");
     }
-    if (c0 > c->ncell_cap) return fail(c, SPH_E_CAPACITY, "supercell table capacity exceeded");
-    c->ncell_tot = c0;
+    c->runs_h = runs;
+    c->ncell_tot = (int)c0;
     HIPTRY(c, hipMemcpyAsync(c->runs_d, c->runs_h.data(), sizeof(RunDev) * c->nruns, hipMemcpyHostToDevice, c->stream));
     HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_cap, c->stream));
     c->binned = false;
@@ -534,11 +536,9 @@ int sph_pool_set_runs(sph_ctx* c, const int32_t* keep, int32_t n_keep, const sph
     }
     std::vector<sph_run_config> cfg;
     std::vector<int64_t> sizes;
-    std::vector<char> fresh;
     for (int k = 0; k < n_keep; ++k) {
         cfg.push_back(c->cfg[keep[k]]);
         sizes.push_back(c->runs_h[keep[k]].np);
-        fresh.push_back(0);
     }
     int64_t cells = 0, parts = 0;
     for (int k = 0; k < n_keep; ++k) {
@@ -554,27 +554,32 @@ int sph_pool_set_runs(sph_ctx* c, const int32_t* keep, int32_t n_keep, const sph
         if (add_sizes[a] < 1) return fail(c, SPH_E_INVALID, "run size must be >= 1");
         cfg.push_back(add[a]);
         sizes.push_back(add_sizes[a]);
-        fresh.push_back(1);
         cells += nc;
         parts += add_sizes[a];
     }
     if (parts > c->cap_particles || cells > c->ncell_cap || (int)cfg.size() > c->cap_runs)
         return fail(c, SPH_E_CAPACITY, "runs do not fit the pool's capacity");
-    // per-run scalars of the kept runs
+    // per-run scalars of the kept runs (a kept run not yet stepped still primes on its first step)
     std::vector<double> old_dt(c->nruns), old_time(c->nruns);
+    std::vector<int32_t> old_prime(c->nruns);
     if (c->nruns) {
         HIPTRY(c, hipMemcpyAsync(old_dt.data(), c->W.dt, sizeof(double) * c->nruns, hipMemcpyDeviceToHost, c->stream));
         HIPTRY(c, hipMemcpyAsync(old_time.data(), c->W.time, sizeof(double) * c->nruns, hipMemcpyDeviceToHost, c->stream));
+        HIPTRY(c, hipMemcpyAsync(old_prime.data(), c->W.prime_flag, sizeof(int32_t) * c->nruns, hipMemcpyDeviceToHost,
+                                 c->stream));
         HIPTRY(c, hipStreamSynchronize(c->stream));
     }
     std::vector<double> dt, time;
+    std::vector<char> fresh;
     for (int k = 0; k < n_keep; ++k) {
         dt.push_back(old_dt[keep[k]]);
         time.push_back(old_time[keep[k]]);
+        fresh.push_back(old_prime[keep[k]] ? 1 : 0);
     }
     for (int a = 0; a < n_add; ++a) {
         dt.push_back(add[a].dt_fixed > 0 ? add[a].dt_fixed : 0.0);
         time.push_back(0.0);
+        fresh.push_back(1);
     }
     // move the kept runs' state into the other buffer set at their new offsets
     const StateBuf& A = c->S[c->cur];

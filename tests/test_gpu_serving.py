@@ -61,6 +61,26 @@ def test_pool_runs_move_and_join():
         assert ca["time"] == c["time"] and ca["dt_next"] == c["dt_next"]
 
 
+def test_kept_run_not_yet_stepped_still_primes():
+    """A run uploaded and then kept by the next set_runs before any step still takes its first step
+    as the priming step (with a fixed dt a lost flag would drift it instead)."""
+    (cfg_a, ps_a), (cfg_b, ps_b) = ic.kelvin_helmholtz(3000, 61), ic.kelvin_helmholtz(4000, 62)
+    cfg_a.dt_fixed = cfg_b.dt_fixed = 1e-4
+    with Pool(20000, 4000, 4, "mixed", True) as pool:
+        pool.set_runs([], [(cfg_a, ps_a.n)])
+        pool.upload(0, ps_a)
+        assert pool.set_runs([0], [(cfg_b, ps_b.n)]) == [1]
+        pool.upload(1, ps_b)
+        pool.step(3)  # the priming step and two steps, for both runs
+        pool.sync()
+        got = [pool.download(r).sorted_by_id() for r in range(2)]
+        cons = pool.conserved()
+    for (cfg, ps), g, c in zip([(cfg_a, ps_a), (cfg_b, ps_b)], got, cons):
+        a, ca = _alone(cfg, ps, 2)
+        _same(a, g)
+        assert ca["time"] == c["time"] == pytest.approx(2e-4, rel=1e-12)
+
+
 def test_pool_rejects_over_capacity():
     cfg, ps = ic.kelvin_helmholtz(3000, 1)
     with Pool(ps.n + 10, 4000, 4, "mixed", True) as pool:
