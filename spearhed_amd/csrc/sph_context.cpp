@@ -687,7 +687,10 @@ int64_t sph_bytes_per_run(const sph_run_config* cfg, int64_t n, int32_t precisio
     const int64_t mhd = cfg->mhd ? 1 : 0;
     const int64_t state = 8 + 2 + pos + 12 + 16 + mhd * 16;
     const int64_t work = 12 + 4 + 4 + 4 + 12 + 8 + 64 + mhd * (12 + 4 + 4 + 16);
-    return n * (2 * state + work) + (int64_t)R.ncell * (12 + 4 * kMaskWords * 64) + 4096;
+    const int64_t blocks = (n + 255) / 256;  // FP64 partial sums (6 doubles) + block->run entry per block
+    const int64_t per_run = 8 + 8 + 4 + 8 * 8 + 4;  // dt, time, cleaning speed, sums, priming flag
+    return n * (2 * state + work) + blocks * (6 * 8 + 4) + (int64_t)R.ncell * (12 + 4 * kMaskWords * 64) + per_run +
+           4096;
 }
 
 int64_t sph_pool_bytes(int64_t max_particles, int64_t max_cells, int32_t max_runs, int32_t precision, int32_t mhd)
@@ -908,7 +911,10 @@ int sph_download(sph_ctx* c, int32_t run, sph_host_soa* d)
         rc |= fetch_col(c, d->dpsidt, c->W.dpsidt + o, n);
         rc |= fetch_col(c, d->divb, c->W.divb + o, n);
     }
-    if (rc) return SPH_E_DEVICE;
+    if (rc) {
+        hipStreamSynchronize(c->stream);  // copies already enqueued into the caller's arrays finish first
+        return SPH_E_DEVICE;
+    }
     HIPTRY(c, hipStreamSynchronize(c->stream));
     if (!c->mixed)
         for (int a = 0; a < 3; ++a)

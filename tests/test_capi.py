@@ -77,3 +77,12 @@ def test_bytes_per_run_is_host_only():
     assert b > 100000 * 100
     bad = capi.make_config(h0=0.0)
     assert lib.sph_bytes_per_run(C.byref(bad), 10, capi.PREC_MIXED) == -1
+    # the admission estimate bounds what the run occupies in a pool sized for it alone (the exact
+    # carve of sph_pool_bytes), and stays within 1% of it
+    for n in (1000, 4096, 100000, 2097152):
+        for mhd in (0, 1):
+            for prec in (capi.PREC_MIXED, capi.PREC_FP32):
+                cfg = capi.make_config(h0=0.01, mhd=mhd)
+                est = lib.sph_bytes_per_run(C.byref(cfg), n, prec)
+                exact = lib.sph_pool_bytes(n, lib.sph_run_cells(C.byref(cfg)), 1, prec, mhd)
+                assert exact <= est <= 1.01 * exact, (n, mhd, prec, est, exact)
