@@ -627,7 +627,7 @@ __global__ __launch_bounds__(64) void k_density(const RunDev* __restrict__ runs,
                     while (msk) {
                         const int ii = __builtin_ctz(msk);
                         msk &= msk - 1;
-                        const float4 p = sp[k + (sb + ii) * K];
+                        const float4 p = sp[k + (int)__umul24((unsigned)(sb + ii), (unsigned)K)];
                         float dx, dy, dz;
                         const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
                         const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
@@ -673,7 +673,10 @@ __device__ __forceinline__ void force_pair(const float4& o0, const float4& o1, c
     const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
     const float rr = r2 * invR;
     const float s_i = kgrad(kern, rr, invR, rr * o2.y, o2.z);
-    const float s_j = kgrad(kern, rr, invR, rr * q2.y, q2.z);
+    // equal smoothing lengths (the records hold h, 1/h and the gradient factor derived from it)
+    // give the same gradient bit for bit: the second evaluation is skipped then
+    float s_j = s_i;
+    if (q2.x != o2.x) s_j = kgrad(kern, rr, invR, rr * q2.y, q2.z);
     const float dvx = o1.x - q1.x, dvy = o1.y - q1.y, dvz = o1.z - q1.z;
     const float vr = dvx * dx + dvy * dy + dvz * dz;
     const float m = q0.w;
@@ -883,7 +886,7 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
                     while (msk) {
                         const int ii = __builtin_ctz(msk);
                         msk &= msk - 1;
-                        const int e = k + (sb + ii) * K;
+                        const int e = k + (int)__umul24((unsigned)(sb + ii), (unsigned)K);
                         float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
                         if constexpr (MHD) q4 = sp[4][e];
                         force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
