@@ -64,3 +64,19 @@ def test_sod_tube_l1(oracle_lib):
     assert l1["vx"] < 0.09, l1
     assert l1["P"] < 0.06, l1
     assert steps > 50
+
+
+def test_full_size_sod_fixture_is_consistent():
+    """The committed 720-step oracle run (tests/golden/make_sod_golden.py): its stored L1 distances
+    follow from its stored profiles and time, and it sits near t = 0.2 with every bin populated."""
+    import os
+
+    import sod_profiles as S
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "sod_full_720.npz"))
+    prof = {k: z[f"prof_{k}"] for k in S.FIELDS}
+    l1 = S.l1_exact(prof, float(z["time"]))
+    for k in S.FIELDS:
+        assert l1[k] == pytest.approx(float(z[f"l1_{k}"]), rel=1e-12)
+    assert 0.2 <= float(z["time"]) < 0.21 and int(z["steps"]) == 720
+    assert np.all(z["count"] > 0)
+    np.testing.assert_allclose(z["centres"], S.CENTRES)
