@@ -13,6 +13,7 @@ import pytest
 import oracle_ref as O
 from golden_io import CASE_NAMES, load_case, load_kernels
 from parity import RTOL, RTOL_GLOBAL
+from spearhed_amd import ic
 
 
 def test_oracle_kernels_match_closed_form_fixture(oracle_lib):
@@ -91,3 +92,21 @@ def test_fp32_oracle_steps_conserve_like_fixture(oracle_lib, name):
     assert np.allclose(e, z["steps_energy"], rtol=RTOL_GLOBAL, atol=0)
     t = np.array([c["time"] for c in cons])
     assert np.allclose(t, z["steps_time"], rtol=1e-5, atol=0)
+
+
+def test_orszag_tang_long_run_fixture_is_consistent():
+    """The committed 100-step oracle run (tests/golden/make_ot_golden.py): its conserved mass is the
+    FP64 sum of the same initial masses, its energies add up to the conserved energy, and its maps
+    have the stated shape."""
+    import os
+
+    import mhd_stats
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "ot_full_100.npz"))
+    _, ps = ic.orszag_tang(n=512, nz=4, seed=2)
+    assert int(z["n_particles"]) == ps.n
+    assert float(z["mass"]) == pytest.approx(float(np.sum(ps.mass.astype(np.float64))), rel=1e-12)
+    total = float(z["e_kin"]) + float(z["e_mag"]) + float(z["e_th"])
+    assert total == pytest.approx(float(z["energy"]), rel=1e-9)
+    for k in ("map_rho", "map_bx", "map_by"):
+        assert z[k].shape == (mhd_stats.NB * mhd_stats.NB,)
+    assert 0.0 < float(z["divb_err"]) < 1e-2 and int(z["steps"]) == 100
