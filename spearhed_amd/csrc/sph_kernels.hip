@@ -1273,25 +1273,45 @@ __global__ __launch_bounds__(256) void k_kick(const RunDev* __restrict__ runs, S
 __global__ __launch_bounds__(256) void k_finalize(const RunDev* __restrict__ runs, WorkBuf W, double dt_host,
                                                   int advance)
 {
-    __shared__ double red[6][256];
+    __shared__ double red[6][4];
     const int r = blockIdx.x;
     const RunDev& R = runs[r];
     double s[6] = {INFINITY, 0, 0, 0, 0, 0};
-    for (int bb = threadIdx.x; bb < R.kb_count; bb += 256) {
-        const double* p = W.partial + (int64_t)(R.kb_first + bb) * 6;
+    const double* base = W.partial + (int64_t)R.kb_first * 6;
+    int bb = threadIdx.x;
+    for (; bb + 768 < R.kb_count; bb += 1024) {  // four partials per round, loads independent
+        double v[4][6];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) v[u][q] = base[(int64_t)(bb + 256 * u) * 6 + q];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            s[0] = fmin(s[0], v[u][0]);
+            for (int q = 1; q < 6; ++q) s[q] += v[u][q];
+        }
+    }
+    for (; bb < R.kb_count; bb += 256) {
+        const double* p = base + (int64_t)bb * 6;
         s[0] = fmin(s[0], p[0]);
         for (int q = 1; q < 6; ++q) s[q] += p[q];
     }
-    for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = s[q];
-    __syncthreads();
-    for (int w = 128; w >= 1; w >>= 1) {
-        if (threadIdx.x < w) {
-            red[0][threadIdx.x] = fmin(red[0][threadIdx.x], red[0][threadIdx.x + w]);
-            for (int q = 1; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
-        }
-        __syncthreads();
+    // butterfly within each wave (every lane ends with the wave's result), then the four waves in order
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        s[0] = fmin(s[0], __shfl_xor(s[0], off, 64));
+#pragma unroll
+        for (int q = 1; q < 6; ++q) s[q] += __shfl_xor(s[q], off, 64);
     }
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 6; ++q) red[q][wid] = s[q];
+    __syncthreads();
     if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            red[0][0] = fmin(red[0][0], red[0][w]);
+            for (int q = 1; q < 6; ++q) red[q][0] += red[q][w];
+        }
         const double dt_used = dt_host > 0.0 ? dt_host : W.dt[r];
         const int priming = W.prime_flag[r];
         double t = W.time[r];
