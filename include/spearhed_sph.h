@@ -165,6 +165,10 @@ int sph_pool_set_runs(sph_ctx* ctx, const int32_t* keep, int32_t n_keep, const s
 const char* sph_last_error(const sph_ctx* ctx);
 int sph_sync(sph_ctx* ctx);
 int sph_get_grid(sph_ctx* ctx, int32_t run, sph_grid_info* out);
+/* Configuration and particle count of run `run` (either pointer may be NULL). */
+int sph_get_run(const sph_ctx* ctx, int32_t run, sph_run_config* cfg_out, int64_t* n_out);
+/* Runs currently in the context (0 for a null context). */
+int32_t sph_run_count(const sph_ctx* ctx);
 int sph_set_tuning(sph_ctx* ctx, const sph_tuning* t);
 int sph_get_tuning(sph_ctx* ctx, sph_tuning* out);
 /* Bytes of HBM a run of n particles needs at a precision (admission by HBM budget). */

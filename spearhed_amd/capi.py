@@ -205,6 +205,8 @@ PROTOTYPES = {
     "sph_last_error": (C.c_char_p, [_CTX]),
     "sph_sync": (C.c_int, [_CTX]),
     "sph_get_grid": (C.c_int, [_CTX, C.c_int32, C.POINTER(GridInfo)]),
+    "sph_get_run": (C.c_int, [_CTX, C.c_int32, C.POINTER(RunConfig), C.POINTER(C.c_int64)]),
+    "sph_run_count": (C.c_int32, [_CTX]),
     "sph_set_tuning": (C.c_int, [_CTX, C.POINTER(Tuning)]),
     "sph_get_tuning": (C.c_int, [_CTX, C.POINTER(Tuning)]),
     "sph_bytes_per_run": (C.c_int64, [C.POINTER(RunConfig), C.c_int64, C.c_int32]),
@@ -258,6 +260,40 @@ def load(path: str = LIB_PATH):
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+OPENPMD_LIB_PATH = os.path.join(HERE, "libspearhed_openpmd.so")
+OPENPMD_PROTOTYPES = {
+    "sph_openpmd_file_name": (C.c_int32, [C.c_char_p, C.c_uint64, C.c_char_p, C.c_int32]),
+    "sph_openpmd_write_soa": (C.c_int, [C.POINTER(HostSoA), C.c_int32, C.c_char_p, C.c_uint64, C.c_double,
+                                        C.c_double]),
+    "sph_write_openpmd": (C.c_int, [_CTX, C.c_int32, C.c_char_p, C.c_uint64]),
+    "sph_openpmd_last_error": (C.c_char_p, []),
+}
+_openpmd = None
+
+
+def load_openpmd(path: str = OPENPMD_LIB_PATH):
+    """Open the openPMD writer (include/spearhed_openpmd.h), after the step library it links to."""
+    global _openpmd
+    if _openpmd is not None:
+        return _openpmd
+    load()
+    if not os.path.exists(path):
+        raise SphError(f"{path} is missing: build it with `python -m spearhed_amd.build` (needs HDF5)")
+    lib = C.CDLL(path)
+    for name, (res, args) in OPENPMD_PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _openpmd = lib
+    return lib
+
+
+def check_openpmd(rc: int, what: str = "") -> None:
+    if rc != SPH_OK:
+        msg = load_openpmd().sph_openpmd_last_error().decode(errors="replace")
+        raise SphError(f"{what}: {ERRORS.get(rc, rc)} {msg}")
 
 
 def check(rc: int, ctx=None, what: str = "") -> None:

@@ -648,6 +648,16 @@ int sph_get_grid(sph_ctx* c, int32_t run, sph_grid_info* o)
     return SPH_OK;
 }
 
+int sph_get_run(const sph_ctx* c, int32_t run, sph_run_config* cfg, int64_t* n)
+{
+    if (!c || run < 0 || run >= c->nruns) return SPH_E_INVALID;
+    if (cfg) *cfg = c->cfg[run];
+    if (n) *n = c->runs_h[run].np;
+    return SPH_OK;
+}
+
+int32_t sph_run_count(const sph_ctx* c) { return c ? c->nruns : 0; }
+
 int sph_set_tuning(sph_ctx* c, const sph_tuning* t)
 {
     if (!c || !t) return SPH_E_INVALID;

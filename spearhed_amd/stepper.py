@@ -122,6 +122,15 @@ class Context:
         check(self.lib.sph_reduce_min_dt(self.h, out), self.h, "sph_reduce_min_dt")
         return np.array(out[:])
 
+    def write_openpmd(self, run: int, pattern: str, iteration: int):
+        """Write run `run` as one openPMD iteration (HDF5, one file per iteration; pattern with %T,
+        e.g. "output_%06T.h5"). Returns the file name."""
+        lib = capi.load_openpmd()
+        capi.check_openpmd(lib.sph_write_openpmd(self.h, run, pattern.encode(), int(iteration)), "sph_write_openpmd")
+        buf = C.create_string_buffer(4096)
+        lib.sph_openpmd_file_name(pattern.encode(), int(iteration), buf, 4096)
+        return buf.value.decode()
+
     def conserved(self) -> list:
         out = (Conserved * self.nruns)()
         check(self.lib.sph_reduce_conserved(self.h, out), self.h, "sph_reduce_conserved")

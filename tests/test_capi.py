@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "spearhed_sph.h")
 
 def declared_entry_points():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(sph_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int32_t|int64_t|const char\*)\s+(sph_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_the_stage_interface():
