@@ -201,6 +201,7 @@ void carve(Carver& cv, sph_ctx* c, int64_t N, int ncell_cap, int nkb, int nruns)
     W.cons = cv.take<double>((int64_t)nruns * 8);
     W.blk_run = cv.take<int32_t>(nkb);
     W.prime_flag = cv.take<int32_t>(nruns);
+    W.move_tab = cv.take<int64_t>((int64_t)nruns * 3);
 }
 
 LaunchCtx lctx(sph_ctx* c)
@@ -581,31 +582,22 @@ int sph_pool_set_runs(sph_ctx* c, const int32_t* keep, int32_t n_keep, const sph
         time.push_back(0.0);
         fresh.push_back(1);
     }
-    // move the kept runs' state into the other buffer set at their new offsets
-    const StateBuf& A = c->S[c->cur];
-    const StateBuf& B = c->S[c->cur ^ 1];
+    // move the kept runs' state into the other buffer set at their new offsets: one table upload
+    // and one kernel for all runs and columns
+    std::vector<int64_t> tab;
     int64_t dst = 0;
     for (int k = 0; k < n_keep; ++k) {
-        const int64_t src = c->runs_h[keep[k]].p0, n = c->runs_h[keep[k]].np;
-        auto mv = [&](auto* to, auto* from) -> hipError_t {
-            if (!to || !from) return hipSuccess;
-            return hipMemcpyAsync(to + dst, from + src, sizeof(*from) * n, hipMemcpyDeviceToDevice, c->stream);
-        };
-        HIPTRY(c, mv(B.id, A.id));
-        HIPTRY(c, mv(B.mask, A.mask));
-        HIPTRY(c, mv(B.species, A.species));
-        for (int a = 0; a < 3; ++a) {
-            HIPTRY(c, mv(B.pos64[a], A.pos64[a]));
-            HIPTRY(c, mv(B.pos32[a], A.pos32[a]));
-            HIPTRY(c, mv(B.vel[a], A.vel[a]));
-            HIPTRY(c, mv(B.B[a], A.B[a]));
-        }
-        HIPTRY(c, mv(B.mass, A.mass));
-        HIPTRY(c, mv(B.h, A.h));
-        HIPTRY(c, mv(B.u, A.u));
-        HIPTRY(c, mv(B.rho, A.rho));
-        HIPTRY(c, mv(B.psi, A.psi));
-        dst += n;
+        tab.push_back(dst);
+        tab.push_back(c->runs_h[keep[k]].p0);
+        tab.push_back(c->runs_h[keep[k]].np);
+        dst += c->runs_h[keep[k]].np;
+    }
+    if (n_keep > 0) {
+        HIPTRY(c, hipMemcpyAsync(c->W.move_tab, tab.data(), sizeof(int64_t) * tab.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPTRY(c, launch_move_runs(c->S[c->cur], c->S[c->cur ^ 1], c->W.move_tab, n_keep, dst, c->mixed ? 1 : 0,
+                                   c->has_mhd ? 1 : 0, c->stream));
+        HIPTRY(c, hipStreamSynchronize(c->stream));  // `tab` is host memory read by the copy
     }
     c->cur ^= 1;
     return layout(c, cfg, sizes, fresh, dt, time);
@@ -698,7 +690,7 @@ int64_t sph_bytes_per_run(const sph_run_config* cfg, int64_t n, int32_t precisio
     const int64_t state = 8 + 2 + pos + 12 + 16 + mhd * 16;
     const int64_t work = 12 + 4 + 4 + 4 + 12 + 8 + 64 + mhd * (12 + 4 + 4 + 16);
     const int64_t blocks = (n + 255) / 256;  // FP64 partial sums (6 doubles) + block->run entry per block
-    const int64_t per_run = 8 + 8 + 4 + 8 * 8 + 4;  // dt, time, cleaning speed, sums, priming flag
+    const int64_t per_run = 8 + 8 + 4 + 8 * 8 + 4 + 3 * 8;  // dt, time, cleaning speed, sums, flag, move entry
     return n * (2 * state + work) + blocks * (6 * 8 + 4) + (int64_t)R.ncell * (12 + 4 * kMaskWords * 64) + per_run +
            4096;
 }

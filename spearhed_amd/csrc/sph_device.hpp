@@ -84,6 +84,7 @@ struct WorkBuf {
     double* cons;     // per run: 8 doubles (mass, px, py, pz, energy, dt_next, time, c_h)
     int32_t* blk_run; // per particle-kernel block: run id
     int32_t* prime_flag; // per run: 1 = the next non-advancing reduction sets the run's first dt
+    int64_t* move_tab;   // per kept run when a pool's run set changes: destination start, source start, count
 };
 
 constexpr float kSentinel = 8.507059e37f;  // FLT_MAX / 4: sentinel^2 overflows to +inf, so r2 < H2 fails

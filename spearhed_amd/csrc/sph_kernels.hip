@@ -1336,6 +1336,44 @@ __global__ __launch_bounds__(256) void k_finalize(const RunDev* __restrict__ run
     }
 }
 
+// Pool compaction: destination particle d belongs to the last kept run whose destination start is
+// <= d; every state column is copied from its source slot.
+__global__ __launch_bounds__(256) void k_move_runs(StateBuf A, StateBuf B, const int64_t* __restrict__ tab, int n_keep,
+                                                   int64_t total, int mixed, int mhd)
+{
+    const int64_t d = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (d >= total) return;
+    int lo = 0, hi = n_keep - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[3 * mid] <= d) lo = mid;
+        else hi = mid - 1;
+    }
+    const int64_t s = tab[3 * lo + 1] + (d - tab[3 * lo]);
+    B.id[d] = A.id[s];
+    B.mask[d] = A.mask[s];
+    B.species[d] = A.species[s];
+    for (int a = 0; a < 3; ++a) {
+        if (mixed) B.pos64[a][d] = A.pos64[a][s];
+        else B.pos32[a][d] = A.pos32[a][s];
+        B.vel[a][d] = A.vel[a][s];
+        if (mhd) B.B[a][d] = A.B[a][s];
+    }
+    B.mass[d] = A.mass[s];
+    B.h[d] = A.h[s];
+    B.u[d] = A.u[s];
+    B.rho[d] = A.rho[s];
+    if (mhd) B.psi[d] = A.psi[s];
+}
+
+hipError_t launch_move_runs(const StateBuf& A, const StateBuf& B, const int64_t* tab, int n_keep, int64_t total,
+                            int mixed, int mhd, hipStream_t s)
+{
+    if (total <= 0 || n_keep <= 0) return hipSuccess;
+    k_move_runs<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(A, B, tab, n_keep, total, mixed, mhd);
+    return hipGetLastError();
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

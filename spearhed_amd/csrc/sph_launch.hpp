@@ -25,6 +25,10 @@ struct LaunchCtx {
 };
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+// Copy the state of n_keep runs from A to B: run k's particles [src, src + count) to [dst, dst + count),
+// with (dst, src, count) at tab[3k..3k+2] on the device, dst ascending from 0 and runs contiguous.
+hipError_t launch_move_runs(const StateBuf& A, const StateBuf& B, const int64_t* tab, int n_keep, int64_t total,
+                            int mixed, int mhd, hipStream_t s);
 hipError_t launch_drift_bin(const LaunchCtx& c, double dt_host, int do_drift);
 hipError_t launch_scan(const LaunchCtx& c);
 hipError_t launch_scatter_gather(const LaunchCtx& c, const StateBuf& Snew);
