@@ -96,26 +96,31 @@ __device__ __forceinline__ bool is_target(uint8_t sp, uint8_t live) { return liv
 __device__ __forceinline__ bool is_source(uint8_t sp, uint8_t live) { return live && sp <= 1; }
 __device__ __forceinline__ bool is_movable(uint8_t sp, uint8_t live) { return live && (sp == 0 || sp == 2); }
 
-// Kernel shape f(q) with W = sigma/h^3 f(q).
+// Kernel shape f(q) with W = sigma/h^3 f(q). The smoothing-kernel functions are written with
+// explicit fused multiply-adds and every other contraction off, so that the instantiation that reads
+// the kernel per run (batches mixing kernels) rounds exactly as the one that fixes it at compile time:
+// a run's results do not depend on the company it is batched with.
 __device__ __forceinline__ float kshape(int kern, float q)
 {
-    if (kern == 2) {  // Wendland C2
-        float t = 1.0f - 0.5f * q;
-        float t2 = t * t;
-        float w = t2 * t2 * (1.0f + 2.0f * q);
+#pragma clang fp contract(off)
+    if (kern == 2) {  // Wendland C2: (1 - q/2)^4 (1 + 2q)
+        const float t = fmaf(-0.5f, q, 1.0f);
+        const float t2 = t * t;
+        const float w = (t2 * t2) * fmaf(2.0f, q, 1.0f);
         return q < 2.0f ? w : 0.0f;
     }
-    if (kern == 0) {  // cubic M4
-        float inner = 1.0f - 1.5f * q * q + 0.75f * q * q * q;
-        float t = 2.0f - q;
-        float outer = 0.25f * t * t * t;
+    if (kern == 0) {  // cubic M4: 1 - 3/2 q^2 + 3/4 q^3 inside, (2 - q)^3 / 4 outside
+        const float inner = fmaf(q * q, fmaf(0.75f, q, -1.5f), 1.0f);
+        const float t = 2.0f - q;
+        const float outer = 0.25f * ((t * t) * t);
         return q < 1.0f ? inner : (q < 2.0f ? outer : 0.0f);
     }
-    // quintic M5
-    float a = 3.0f - q, b = 2.0f - q, c = 1.0f - q;
-    float a5 = a * a * a * a * a, b5 = b * b * b * b * b, c5 = c * c * c * c * c;
-    if (q < 1.0f) return a5 - 6.0f * b5 + 15.0f * c5;
-    if (q < 2.0f) return a5 - 6.0f * b5;
+    // quintic M5: (3 - q)^5 - 6 (2 - q)^5 + 15 (1 - q)^5, terms dropped beyond their support
+    const float a = 3.0f - q, b = 2.0f - q, c = 1.0f - q;
+    const float a2 = a * a, b2 = b * b, c2 = c * c;
+    const float a5 = (a2 * a2) * a, b5 = (b2 * b2) * b, c5 = (c2 * c2) * c;
+    if (q < 1.0f) return fmaf(15.0f, c5, fmaf(-6.0f, b5, a5));
+    if (q < 2.0f) return fmaf(-6.0f, b5, a5);
     if (q < 3.0f) return a5;
     return 0.0f;
 }
@@ -123,23 +128,25 @@ __device__ __forceinline__ float kshape(int kern, float q)
 // Gradient scalar s with grad_i W_ij = s r_ij (gfac includes sigma and the h powers).
 __device__ __forceinline__ float kgrad(int kern, float rr, float invR, float q, float gfac)
 {
+#pragma clang fp contract(off)
     float s;
     if (kern == 2) {
-        float t = 1.0f - 0.5f * q;
-        s = gfac * (t * t * t);
+        const float t = fmaf(-0.5f, q, 1.0f);
+        s = gfac * ((t * t) * t);
         s = q < 2.0f ? s : 0.0f;
     } else if (kern == 0) {
-        float gi = 2.25f * q * q - 3.0f * q;
-        float t = 2.0f - q;
-        float go = -0.75f * t * t;
-        float g = q < 1.0f ? gi : (q < 2.0f ? go : 0.0f);
-        s = gfac * g * invR;
+        const float gi = q * fmaf(2.25f, q, -3.0f);
+        const float t = 2.0f - q;
+        const float go = -0.75f * (t * t);
+        const float g = q < 1.0f ? gi : (q < 2.0f ? go : 0.0f);
+        s = (gfac * g) * invR;
     } else {
-        float a = 3.0f - q, b = 2.0f - q, c = 1.0f - q;
-        float a4 = a * a * a * a, b4 = b * b * b * b, c4 = c * c * c * c;
-        float g = q < 1.0f ? (-5.0f * a4 + 30.0f * b4 - 75.0f * c4)
-                           : (q < 2.0f ? (-5.0f * a4 + 30.0f * b4) : (q < 3.0f ? -5.0f * a4 : 0.0f));
-        s = gfac * g * invR;
+        const float a = 3.0f - q, b = 2.0f - q, c = 1.0f - q;
+        const float a2 = a * a, b2 = b * b, c2 = c * c;
+        const float a4 = a2 * a2, b4 = b2 * b2, c4 = c2 * c2;
+        const float g = q < 1.0f ? fmaf(-75.0f, c4, fmaf(30.0f, b4, -5.0f * a4))
+                                 : (q < 2.0f ? fmaf(30.0f, b4, -5.0f * a4) : (q < 3.0f ? -5.0f * a4 : 0.0f));
+        s = (gfac * g) * invR;
     }
     return rr > 0.0f ? s : 0.0f;
 }

@@ -279,3 +279,23 @@ def test_forces_parity_every_launch_shape(mode, case, prec):
     g = gpu_stages(cfg, ps, GPREC[prec], tuning=MODES[mode])
     o = oracle_stages(cfg, ps, prec)
     compare_forces(g, o, bool(cfg.mhd))
+
+
+def test_mixed_kernel_batch():
+    """Runs with different smoothing kernels in one context take the launch instantiation that reads
+    the kernel per run: each run meets the oracle contract, and equals the same run alone."""
+    runs = [ic.mhd_turbulence(n=10, seed=31 + k, kernel=kern)
+            for k, kern in enumerate([capi.KERNEL_CUBIC, capi.KERNEL_QUINTIC, capi.KERNEL_WENDLAND])]
+    with Context([c for c, _ in runs], [p.n for _, p in runs], precision="mixed") as ctx:
+        for r, (_, p) in enumerate(runs):
+            ctx.upload(r, p)
+        ctx.resort()
+        ctx.density()
+        ctx.forces()
+        ctx.sync()
+        mixed = [ctx.download(r).sorted_by_id() for r in range(len(runs))]
+    for (cfg, ps), g in zip(runs, mixed):
+        compare_forces(g, oracle_stages(cfg, ps, "mixed"), mhd=True)
+        alone = gpu_stages(cfg, ps, "mixed")
+        for f in ("rho", "nnb", "dvdt", "dudt", "dbdt", "divb", "dpsidt", "vsig"):
+            assert np.array_equal(getattr(g, f), getattr(alone, f)), (cfg.kernel, f)
