@@ -299,3 +299,25 @@ def test_mixed_kernel_batch():
         alone = gpu_stages(cfg, ps, "mixed")
         for f in ("rho", "nnb", "dvdt", "dudt", "dbdt", "divb", "dpsidt", "vsig"):
             assert np.array_equal(getattr(g, f), getattr(alone, f)), (cfg.kernel, f)
+
+
+def test_hydro_run_batched_with_mhd_runs_equals_alone():
+    """A hydro run in a context that carries MHD fields (another run is MHD) takes the MHD launch
+    instantiation with its MHD terms off; its results equal the same run alone in a hydro context."""
+    runs = [ic.kelvin_helmholtz(4000, 71), ic.sod_tube(seed=7, nx_left=32, cross=4)]
+    got, alone = [], []
+    with Context([c for c, _ in runs], [p.n for _, p in runs], precision="mixed") as ctx:
+        for r, (_, ps) in enumerate(runs):
+            ctx.upload(r, ps)
+        ctx.step(3)
+        ctx.sync()
+        got = [ctx.download(r).sorted_by_id() for r in range(len(runs))]
+    for cfg, ps in runs:
+        with Context([cfg], [ps.n], precision="mixed") as ctx:
+            ctx.upload(0, ps)
+            ctx.step(3)
+            ctx.sync()
+            alone.append(ctx.download(0).sorted_by_id())
+    for a, b in zip(alone, got):
+        for f in ("pos", "vel", "u", "rho", "dvdt", "dudt", "vsig", "nnb"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), f
