@@ -14,6 +14,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -245,8 +246,17 @@ public:
         ctx_.prime();
     }
 
+    // Called after every step whose number is a multiple of `period` (the reference's plugin
+    // notification, PluginConnector::setNotificationPeriod as OpenPMDPlugin.x.cpp uses it), e.g.
+    // with sph_write_openpmd(ctx.handle(), run, "output_%06T.h5", step) from spearhed_openpmd.h.
+    void setOutput(uint32_t period, std::function<void(Context&, uint32_t)> notify)
+    {
+        period_ = period;
+        notify_ = std::move(notify);
+    }
+
     // One step through the individual stage calls (what sph_step does in one call).
-    void runOneStep(uint32_t /*step*/)
+    void runOneStep(uint32_t step)
     {
         PushVelocity{}(ctx_);
         UpdateVolumes{}(ctx_);
@@ -254,6 +264,7 @@ public:
         UpdateHydroForces{}(ctx_);
         EulerIntegrate{}(ctx_);
         ctx_.sync();
+        if (period_ > 0 && notify_ && step % period_ == 0) notify_(ctx_, step);
     }
 
     // n steps through sph_step (identical arithmetic, fewer host calls).
@@ -265,6 +276,8 @@ public:
 
 private:
     Context ctx_;
+    uint32_t period_ = 0;
+    std::function<void(Context&, uint32_t)> notify_;
 };
 
 }  // namespace spearhed::amd

@@ -331,20 +331,37 @@ class EulerIntegrate:
 
 
 class Simulation:
-    """Simulation::runOneStep (include/spearhed/control/Simulation.x.cpp:196-310) over a Context."""
+    """Simulation::runOneStep (include/spearhed/control/Simulation.x.cpp:196-310) over a Context.
 
-    def __init__(self, ctx: Context):
+    openpmd_period > 0 writes every run as an openPMD iteration after each step whose number is a
+    multiple of the period, as the reference's OpenPMDPlugin (openPMD.period / openPMD.file options,
+    include/spearhed/plugins/openPMD/OpenPMDPlugin.x.cpp) does; with several runs in the context, run
+    r goes to the pattern with "_run<r>" before its extension."""
+
+    def __init__(self, ctx: Context, openpmd_period: int = 0, openpmd_file: str = "output_%06T.h5"):
         self.ctx = ctx
         self.push = PushVelocity()
         self.update_volumes = UpdateVolumes()
         self.update_density = UpdateDensity()
         self.update_hydro = UpdateHydroForces()
         self.euler = EulerIntegrate()
+        self.openpmd_period = int(openpmd_period)
+        self.openpmd_file = openpmd_file
+        self.written = []  # files written by the output hook
 
-    def run_one_step(self, dt: float = 0.0):
+    def _pattern(self, run: int) -> str:
+        if self.ctx.nruns == 1:
+            return self.openpmd_file
+        stem, dot, ext = self.openpmd_file.rpartition(".")
+        return f"{stem}_run{run}.{ext}" if dot else f"{self.openpmd_file}_run{run}"
+
+    def run_one_step(self, dt: float = 0.0, step: int = None):
         self.push(self.ctx, dt)
         self.update_volumes(self.ctx)
         self.update_density(self.ctx)
         self.update_hydro(self.ctx)
         self.euler(self.ctx, dt)
         self.ctx.sync()
+        if self.openpmd_period > 0 and step is not None and step % self.openpmd_period == 0:
+            for r in range(self.ctx.nruns):
+                self.written.append(self.ctx.write_openpmd(r, self._pattern(r), step))

@@ -107,7 +107,16 @@ static int gpu_checks()
     Simulation staged({cfg}, {N}), fused({cfg}, {N});
     staged.init(ic1);
     fused.init(ic2);
+    // the output hook fires after steps 0 and 2 (period 2), with the context the step ran on
+    std::vector<uint32_t> notified;
+    staged.setOutput(2, [&](Context& ctx, uint32_t step) {
+        if (&ctx == &staged.context()) notified.push_back(step);
+    });
     for (uint32_t s = 0; s < 3; ++s) staged.runOneStep(s);
+    if (notified != std::vector<uint32_t>{0, 2}) {
+        std::printf("{\"error\": \"output hook fired at the wrong steps\"}\n");
+        return 1;
+    }
     fused.run(3);
     ParticleColumns a = staged.context().download(0), b = fused.context().download(0);
     auto ia = index_by_id(a), ib = index_by_id(b);

@@ -82,3 +82,24 @@ def test_mhd_run_written_with_its_magnetic_records(tmp_path):
     assert np.array_equal(r["id"], g.id)
     assert np.array_equal(r["magnetic_field"], g.B) and np.array_equal(r["psi"], g.psi)
     assert t[f"/data/2/particles/fluid/magnetic_field"]["attrs"]["unitDimension"]["value"] == [0, 1, -2, -1, 0, 0, 0]
+
+
+def test_simulation_writes_openpmd_every_period(tmp_path):
+    """stepper.Simulation with openpmd_period = 2 (the reference's openPMD.period option): after the
+    steps whose number is a multiple of 2, one file per run, holding that step's state."""
+    from spearhed_amd.stepper import Simulation
+    runs = [ic.kelvin_helmholtz(3000, 81), ic.kelvin_helmholtz(4000, 82)]
+    with Context([c for c, _ in runs], [p.n for _, p in runs], precision="mixed") as ctx:
+        for r, (_, ps) in enumerate(runs):
+            ctx.upload(r, ps)
+        ctx.prime()
+        sim = Simulation(ctx, openpmd_period=2, openpmd_file=str(tmp_path / "kh_%04T.h5"))
+        for step in range(1, 5):
+            sim.run_one_step(step=step)
+        final = [ctx.download(r) for r in range(len(runs))]
+    names = sorted(os.path.basename(f) for f in sim.written)
+    assert names == ["kh_0002_run0.h5", "kh_0002_run1.h5", "kh_0004_run0.h5", "kh_0004_run1.h5"]
+    for r, g in enumerate(final):
+        t = openpmd_io.read(str(tmp_path / f"kh_0004_run{r}.h5"))
+        f = _species(t, 4, "fluid")
+        assert np.array_equal(f["id"], g.id[g.mask == 1]) and np.array_equal(f["position"], g.pos[:, g.mask == 1])
