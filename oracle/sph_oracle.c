@@ -114,6 +114,59 @@ int orc_sort(int64_t n, const uint32_t* key, const uint64_t* id, int32_t ncell, 
     return 0;
 }
 
+/* Region layout of the region traversal: regions are the blocks of block^3 supercells that hold
+ * particles at set-up (SetupInterface.hpp:81-96 makes one region per set-up block; here the blocks
+ * are supercell-sized so the brute-force search has compact regions to test). A region's particles,
+ * in id order, fill its 64-slot frames from region_start[r]; slot[i] is particle i's slot.
+ * region_start holds cap_regions + 1 entries. */
+int orc_regions_layout(const sph_run_config* cfg, int32_t prec, int64_t n, double* const pos[3], const uint64_t* id,
+                       int32_t block, int64_t* slot, int64_t* region_start, int32_t cap_regions, int32_t* n_regions,
+                       int64_t* n_slots)
+{
+    orc_grid g;
+    if (orc_make_grid(cfg, &g) || block < 1 || n < 0) return -1;
+    int nb[3];
+    int64_t nblk = 1;
+    for (int a = 0; a < 3; ++a) {
+        nb[a] = (g.n[a] + block - 1) / block;
+        nblk *= nb[a];
+    }
+    if (nblk >= 0x7fffffffLL) return -1;
+    uint32_t* key = (uint32_t*)malloc((size_t)n * 4 + 4);
+    int64_t* perm = (int64_t*)malloc((size_t)n * 8 + 8);
+    uint32_t* bs = (uint32_t*)malloc(((size_t)nblk + 1) * 4);
+    int rc = (key && perm && bs) ? 0 : -2;
+    if (!rc) rc = orc_bin(cfg, prec, n, pos, key);
+    if (!rc) {
+        for (int64_t i = 0; i < n; ++i) {
+            const int cx = (int)(key[i] % (uint32_t)g.n[0]);
+            const int cy = (int)((key[i] / (uint32_t)g.n[0]) % (uint32_t)g.n[1]);
+            const int cz = (int)(key[i] / ((uint32_t)g.n[0] * (uint32_t)g.n[1]));
+            key[i] = (uint32_t)(cx / block + nb[0] * (cy / block + nb[1] * (cz / block)));
+        }
+        rc = orc_sort(n, key, id, (int32_t)nblk, perm, bs);
+    }
+    int32_t R = 0;
+    int64_t s = 0;
+    for (int64_t b = 0; !rc && b < nblk; ++b) {
+        const int64_t cnt = (int64_t)bs[b + 1] - (int64_t)bs[b];
+        if (cnt == 0) continue;
+        if (R >= cap_regions) { rc = -1; break; }
+        region_start[R++] = s;
+        for (int64_t k = 0; k < cnt; ++k) slot[perm[bs[b] + k]] = s + k;
+        s += (cnt + 63) / 64 * 64;
+    }
+    if (!rc) {
+        region_start[R] = s;
+        *n_regions = R;
+        *n_slots = s;
+    }
+    free(key);
+    free(perm);
+    free(bs);
+    return rc;
+}
+
 #include "sph_oracle_common.h"
 
 #define REAL float

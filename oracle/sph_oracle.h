@@ -98,11 +98,21 @@ int orc_sort(int64_t n, const uint32_t* key, const uint64_t* id, int32_t ncell, 
     int orc_step_##S(const sph_run_config* cfg, int32_t prec, orc_state* st, int32_t n_steps,      \
                      sph_conserved* out);                                                           \
     int orc_brute_counts_##S(const sph_run_config* cfg, int32_t prec, orc_state* st);              \
+    int orc_regions_step_##S(const sph_run_config* cfg, int32_t prec, orc_state* st, int32_t n_regions, \
+                             const int64_t* region_start, int32_t n_steps, int64_t* csr_entries,   \
+                             sph_conserved* out);                                                   \
     double orc_kernel_w_##S(int32_t kernel, double r, double h);                                    \
     double orc_kernel_dwdr_##S(int32_t kernel, double r, double h);
 
 ORC_DECL(f32)
 ORC_DECL(f64)
+
+/* The reference's region/frame traversal (the CPU baseline's reference-structured algorithm):
+ * orc_regions_layout assigns each particle a slot in 64-slot frames of supercell-block regions;
+ * orc_regions_step_S steps a state laid out that way (n_steps = 0: the priming evaluation). */
+int orc_regions_layout(const sph_run_config* cfg, int32_t prec, int64_t n, double* const pos[3], const uint64_t* id,
+                       int32_t block, int64_t* slot, int64_t* region_start, int32_t cap_regions, int32_t* n_regions,
+                       int64_t* n_slots);
 
 int orc_num_threads(void);
 

@@ -200,6 +200,38 @@ static inline REAL FN(geom)(const REAL pi[3], REAL* const pc[3], int64_t j, cons
     return RFMA(d[2], d[2], RFMA(d[1], d[1], d[0] * d[0]));
 }
 
+/* A run of source slots [b, e) staged with one shift: a stencil supercell (binned traversal) or a
+ * neighbour region under one periodic image (region traversal). `own` marks the target's own
+ * supercell / region with no shift, where the target itself is skipped (the reference's isSelf,
+ * ParticleParticleInteraction.hpp:299). */
+typedef struct FN(span) {
+    int64_t b, e;
+    int own;
+    REAL sh[3];
+} FN(span);
+
+/* The 27-supercell stencil of one cell as spans (fewer at a non-periodic wall). */
+static int FN(cell_spans)(const orc_grid* g, int32_t prec, const sph_run_config* cfg, const uint32_t* cell_start,
+                          int cell, FN(span)* sp)
+{
+    int c[3];
+    cell_coords(g, cell, c);
+    int n = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                int d[3] = {dx, dy, dz};
+                int cc;
+                if (FN(nb_cell)(g, prec, cfg, c, d, &cc, sp[n].sh)) {
+                    sp[n].b = cell_start[cc];
+                    sp[n].e = cell_start[cc + 1];
+                    sp[n].own = (dx == 0 && dy == 0 && dz == 0);
+                    ++n;
+                }
+            }
+    return n;
+}
+
 int FN(orc_permute)(orc_state* st, const int64_t* perm)
 {
     int64_t n = st->n;
@@ -291,6 +323,61 @@ static void FN(prims)(const sph_run_config* cfg, const orc_state* st, FN(prim)* 
     }
 }
 
+/* Density of target i summed over the source spans (DensitySummation.hpp:50-138 semantics): the
+ * self term plus m_j W(r_ij, h_i) over sources within H, with the neighbour count. */
+static void FN(density_one)(const sph_run_config* cfg, orc_state* st, REAL* const pc[3], REAL H2, REAL f0, int64_t i,
+                            const FN(span)* sp, int nsp)
+{
+    REAL* rho = RA(st->rho);
+    const REAL* m = RA(st->mass);
+    REAL invh, wfac, gfac;
+    FN(kfactors)(cfg->kernel, RA(st->h)[i], &invh, &wfac, &gfac);
+    REAL pi[3] = {pc[0][i], pc[1][i], pc[2][i]};
+    REAL acc = (REAL)0;
+    double sc = 0.0;
+    uint32_t cnt = 0;
+    for (int k = 0; k < nsp; ++k) {
+        const int own = sp[k].own;
+        for (int64_t j = sp[k].b; j < sp[k].e; ++j) {
+            if (!is_source(st->species[j], st->mask[j])) continue;
+            REAL d[3];
+            REAL r2 = FN(geom)(pi, pc, j, sp[k].sh, d);
+            if (!(r2 < H2)) continue;
+            if (own && j == i) continue;
+            ++cnt;
+            REAL invR = (REAL)0, rr = (REAL)0;
+            if (r2 > (REAL)0) {
+                invR = (REAL)1 / (REAL)sqrt(r2);
+                rr = r2 * invR;
+            }
+            REAL q = rr * invh;
+            REAL t = m[j] * FN(kshape)(cfg->kernel, q);
+            acc += t;
+            sc += fabs((double)t);
+        }
+    }
+    REAL self = m[i] * f0;
+    rho[i] = wfac * (acc + self);
+    if (st->nnb) st->nnb[i] = cnt;
+    if (st->sc_rho) st->sc_rho[i] = fabs((double)wfac) * (sc + fabs((double)self));
+}
+
+/* Cleaning speed after a density pass: max fast speed over targets (new; Tricco & Price 2012
+ * sec. 2); 0 without cleaning. */
+static double FN(clean_speed)(const sph_run_config* cfg, const orc_state* st)
+{
+    double chmax = 0.0;
+    if (cfg->mhd && cfg->cleaning) {
+        int64_t n = st->n;
+        FN(prim)* pr = (FN(prim)*)malloc((size_t)n * sizeof(FN(prim)) + 8);
+        FN(prims)(cfg, st, pr);
+        for (int64_t i = 0; i < n; ++i)
+            if (is_target(st->species[i], st->mask[i]) && (double)pr[i].c > chmax) chmax = (double)pr[i].c;
+        free(pr);
+    }
+    return chmax;
+}
+
 /* Density summation over the 27-supercell stencil (DensitySummation.hpp:50-138 semantics). */
 int FN(orc_density)(const sph_run_config* cfg, int32_t prec, orc_state* st, const uint32_t* cell_start)
 {
@@ -302,78 +389,209 @@ int FN(orc_density)(const sph_run_config* cfg, int32_t prec, orc_state* st, cons
     FN(pair_coords)(&g, prec, st, pc);
     const REAL H2 = (prec == ORC_PREC_F64) ? (REAL)g.H2d : (REAL)g.H2;
     const REAL f0 = FN(kshape)(cfg->kernel, (REAL)0);
-    REAL* rho = RA(st->rho);
-    const REAL* m = RA(st->mass);
-    double chmax = 0.0;
 
 #pragma omp parallel for schedule(dynamic, 16)
     for (int cell = 0; cell < g.ncell; ++cell) {
-        int c[3];
-        cell_coords(&g, cell, c);
-        int ncc = 0, ccs[27];
-        REAL sh[27][3];
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    int d[3] = {dx, dy, dz};
-                    if (FN(nb_cell)(&g, prec, cfg, c, d, &ccs[ncc], sh[ncc])) {
-                        if (dx == 0 && dy == 0 && dz == 0) ccs[ncc] |= 0x80000000u; /* own cell */
-                        ++ncc;
-                    }
-                }
+        FN(span) sp[27];
+        const int nsp = FN(cell_spans)(&g, prec, cfg, cell_start, cell, sp);
         for (int64_t i = cell_start[cell]; i < (int64_t)cell_start[cell + 1]; ++i) {
             if (!is_target(st->species[i], st->mask[i])) {
                 if (st->nnb) st->nnb[i] = 0;
                 continue;
             }
-            REAL invh, wfac, gfac;
-            FN(kfactors)(cfg->kernel, RA(st->h)[i], &invh, &wfac, &gfac);
-            REAL pi[3] = {pc[0][i], pc[1][i], pc[2][i]};
-            REAL acc = (REAL)0;
-            double sc = 0.0;
-            uint32_t cnt = 0;
-            for (int k = 0; k < ncc; ++k) {
-                int own = (ccs[k] & 0x80000000u) != 0;
-                int cc = ccs[k] & 0x7fffffff;
-                for (int64_t j = cell_start[cc]; j < (int64_t)cell_start[cc + 1]; ++j) {
-                    if (!is_source(st->species[j], st->mask[j])) continue;
-                    REAL d[3];
-                    REAL r2 = FN(geom)(pi, pc, j, sh[k], d);
-                    if (!(r2 < H2)) continue;
-                    if (own && j == i) continue;
-                    ++cnt;
-                    REAL invR = (REAL)0, rr = (REAL)0;
-                    if (r2 > (REAL)0) {
-                        invR = (REAL)1 / (REAL)sqrt(r2);
-                        rr = r2 * invR;
-                    }
-                    REAL q = rr * invh;
-                    REAL t = m[j] * FN(kshape)(cfg->kernel, q);
-                    acc += t;
-                    sc += fabs((double)t);
-                }
-            }
-            REAL self = m[i] * f0;
-            rho[i] = wfac * (acc + self);
-            if (st->nnb) st->nnb[i] = cnt;
-            if (st->sc_rho) st->sc_rho[i] = fabs((double)wfac) * (sc + fabs((double)self));
+            FN(density_one)(cfg, st, pc, H2, f0, i, sp, nsp);
         }
     }
     for (int a = 0; a < 3; ++a) free(pc[a]);
-
-    /* cleaning speed: max fast speed over targets (new; Tricco & Price 2012 sec. 2) */
-    if (cfg->mhd && cfg->cleaning) {
-        FN(prim)* pr = (FN(prim)*)malloc((size_t)n * sizeof(FN(prim)) + 8);
-        FN(prims)(cfg, st, pr);
-        for (int64_t i = 0; i < n; ++i)
-            if (is_target(st->species[i], st->mask[i]) && (double)pr[i].c > chmax) chmax = (double)pr[i].c;
-        free(pr);
-    }
-    st->c_h = chmax;
+    st->c_h = FN(clean_speed)(cfg, st);
     return 0;
 }
 
-/* Force pass: HydroInteraction (HydroForces.hpp:139-175) + Monaghan viscosity + SPMHD + cleaning. */
+/* Inputs of the force pass shared by all targets. */
+typedef struct FN(fctx) {
+    REAL* pc[3];
+    const FN(prim)* pr;
+    REAL H2, alpha, beta, borve, ch, csig;
+    int kern, mhd, clean;
+} FN(fctx);
+
+static void FN(fctx_init)(const sph_run_config* cfg, const orc_state* st, REAL H2, REAL* const pc[3],
+                          const FN(prim)* pr, FN(fctx)* F)
+{
+    for (int a = 0; a < 3; ++a) F->pc[a] = pc[a];
+    F->pr = pr;
+    F->H2 = H2;
+    F->kern = cfg->kernel;
+    F->mhd = cfg->mhd;
+    F->clean = cfg->mhd && cfg->cleaning;
+    F->alpha = (REAL)cfg->alpha_visc;
+    F->beta = (REAL)cfg->beta_visc;
+    F->borve = (cfg->mhd && cfg->borve) ? (REAL)1 : (REAL)0;
+    F->ch = (REAL)st->c_h;
+    F->csig = (REAL)cfg->clean_sigma;
+}
+
+/* Forces on target i over the source spans: HydroInteraction (HydroForces.hpp:139-175) + Monaghan
+ * viscosity + SPMHD + cleaning. */
+static void FN(force_one)(const FN(fctx)* F, orc_state* st, int64_t i, const FN(span)* sp, int nsp)
+{
+    REAL* const* pc = F->pc;
+    const FN(prim)* pr = F->pr;
+    const REAL H2 = F->H2;
+    const int kern = F->kern, mhd = F->mhd, clean = F->clean;
+    const REAL alpha = F->alpha, beta = F->beta, borve = F->borve, ch = F->ch, csig = F->csig;
+    const REAL* m = RA(st->mass);
+    const REAL* hh = RA(st->h);
+    const REAL* rho = RA(st->rho);
+    const REAL *vx = RA(st->vel[0]), *vy = RA(st->vel[1]), *vz = RA(st->vel[2]);
+    const REAL *Bx = mhd ? RA(st->B[0]) : NULL, *By = mhd ? RA(st->B[1]) : NULL,
+               *Bz = mhd ? RA(st->B[2]) : NULL;
+    const FN(prim) P_i = pr[i];
+    REAL pi[3] = {pc[0][i], pc[1][i], pc[2][i]};
+    REAL bi[3] = {0, 0, 0};
+    if (mhd) { bi[0] = Bx[i]; bi[1] = By[i]; bi[2] = Bz[i]; }
+    REAL ax = 0, ay = 0, az = 0, DV = 0, AVU = 0, vmax = 0;
+    REAL Ti = 0, Tj = 0, TBx = 0, TBy = 0, TBz = 0, VBx = 0, VBy = 0, VBz = 0, DB = 0;
+    REAL PSx = 0, PSy = 0, PSz = 0;
+    double sa = 0, sdu = 0, sdb = 0, sdiv = 0, sdvv = 0;
+    for (int k = 0; k < nsp; ++k) {
+        const int own = sp[k].own;
+        for (int64_t j = sp[k].b; j < sp[k].e; ++j) {
+            if (!is_source(st->species[j], st->mask[j])) continue;
+            REAL d[3];
+            REAL r2 = FN(geom)(pi, pc, j, sp[k].sh, d);
+            if (!(r2 < H2)) continue;
+            if (own && j == i) continue;
+            const FN(prim) P_j = pr[j];
+            REAL invR = (REAL)0, rr = (REAL)0;
+            if (r2 > (REAL)0) {
+                invR = (REAL)1 / (REAL)sqrt(r2);
+                rr = r2 * invR;
+            }
+            REAL s_i = FN(kgrad)(kern, rr, invR, rr * P_i.invh, P_i.gfac);
+            REAL s_j = FN(kgrad)(kern, rr, invR, rr * P_j.invh, P_j.gfac);
+            REAL dvx = vx[i] - vx[j], dvy = vy[i] - vy[j], dvz = vz[i] - vz[j];
+            REAL vr = dvx * d[0] + dvy * d[1] + dvz * d[2];
+            REAL ms_i = m[j] * s_i, ms_j = m[j] * s_j;
+            /* pressure (+ magnetic pressure) gradient, symmetric form */
+            REAL fiso = P_i.pr * ms_i + P_j.pr * ms_j;
+            REAL cax = -fiso * d[0], cay = -fiso * d[1], caz = -fiso * d[2];
+            DV += ms_i * vr;
+            /* scales use the magnitudes before cancellation: sum_a |dv_a d_a| for v.r (and
+             * sum_a |B_a d_a| for B.r below), the bound of an FP32 evaluation in any order */
+            const double vdn = fabs((double)dvx * (double)d[0]) + fabs((double)dvy * (double)d[1])
+                               + fabs((double)dvz * (double)d[2]);
+            sdvv += fabs((double)ms_i) * vdn;
+            sdu += fabs((double)(P_i.ph * ms_i)) * vdn;
+            /* signal velocity */
+            REAL w = vr * invR;
+            REAL vs = P_i.c + P_j.c - beta * (w < (REAL)0 ? w : (REAL)0);
+            if (vs > vmax) vmax = vs;
+            /* artificial viscosity */
+            if (alpha > (REAL)0 && vr < (REAL)0) {
+                REAL hb = (REAL)0.5 * (hh[i] + hh[j]);
+                REAL mu = hb * vr / (r2 + (REAL)0.01 * hb * hb);
+                REAL cb = (REAL)0.5 * (P_i.c + P_j.c);
+                REAL rb = (REAL)0.5 * (rho[i] + rho[j]);
+                REAL Pi = (beta * mu - alpha * cb) * mu / rb;
+                REAL gv = Pi * (REAL)0.5 * (ms_i + ms_j);
+                cax -= gv * d[0];
+                cay -= gv * d[1];
+                caz -= gv * d[2];
+                AVU += gv * vr;
+                sdu += fabs((double)gv) * vdn;
+            }
+            ax += cax;
+            ay += cay;
+            az += caz;
+            double cnorm = sqrt((double)cax * (double)cax + (double)cay * (double)cay + (double)caz * (double)caz);
+            if (mhd) {
+                REAL Bir = bi[0] * d[0] + bi[1] * d[1] + bi[2] * d[2];
+                REAL Bjr = Bx[j] * d[0] + By[j] * d[1] + Bz[j] * d[2];
+                REAL ti = Bir * P_i.bdr * ms_i;
+                REAL tj = Bjr * P_j.bdr * ms_j;
+                Ti += ti;
+                Tj += tj;
+                TBx += tj * Bx[j];
+                TBy += tj * By[j];
+                TBz += tj * Bz[j];
+                REAL wv = ms_i * Bir;
+                VBx += wv * dvx;
+                VBy += wv * dvy;
+                VBz += wv * dvz;
+                REAL db = ms_i * (Bir - Bjr);
+                DB += db;
+                double bn_i = sqrt((double)bi[0] * (double)bi[0] + (double)bi[1] * (double)bi[1] + (double)bi[2] * (double)bi[2]);
+                double bn_j = sqrt((double)Bx[j] * (double)Bx[j] + (double)By[j] * (double)By[j] + (double)Bz[j] * (double)Bz[j]);
+                const double bdi = fabs((double)bi[0] * (double)d[0]) + fabs((double)bi[1] * (double)d[1])
+                                   + fabs((double)bi[2] * (double)d[2]);
+                const double bdj = fabs((double)Bx[j] * (double)d[0]) + fabs((double)By[j] * (double)d[1])
+                                   + fabs((double)Bz[j] * (double)d[2]);
+                const double ati = fabs((double)(P_i.bdr * ms_i)) * bdi;
+                const double atj = fabs((double)(P_j.bdr * ms_j)) * bdj;
+                cnorm += ati * bn_i + atj * bn_j + (double)borve * (ati + atj) * bn_i;
+                double dvn = sqrt((double)dvx * (double)dvx + (double)dvy * (double)dvy + (double)dvz * (double)dvz);
+                sdb += (fabs((double)ms_i) * bdi * dvn + bn_i * fabs((double)ms_i) * vdn) / (double)rho[i];
+                sdiv += fabs((double)ms_i) * (bdi + bdj) / (double)rho[i];
+                if (clean) {
+                    REAL fps = P_i.psr * ms_i + P_j.psr * ms_j;
+                    PSx += fps * d[0];
+                    PSy += fps * d[1];
+                    PSz += fps * d[2];
+                    double dn = sqrt((double)d[0] * (double)d[0] + (double)d[1] * (double)d[1] + (double)d[2] * (double)d[2]);
+                    sdb += (double)rho[i] * (fabs((double)(P_i.psr * ms_i)) + fabs((double)(P_j.psr * ms_j))) * dn;
+                }
+            }
+            sa += cnorm;
+        }
+    }
+    REAL invrho = (REAL)1 / rho[i];
+    if (mhd) {
+        REAL tcoef = Ti - borve * (Ti + Tj);
+        ax = ax + TBx + bi[0] * tcoef;
+        ay = ay + TBy + bi[1] * tcoef;
+        az = az + TBz + bi[2] * tcoef;
+    }
+    RA(st->dvdt[0])[i] = ax;
+    RA(st->dvdt[1])[i] = ay;
+    RA(st->dvdt[2])[i] = az;
+    RA(st->dudt)[i] = P_i.ph * DV + (REAL)0.5 * AVU;
+    REAL vsg = (REAL)2 * P_i.c;
+    if (vmax > vsg) vsg = vmax;
+    RA(st->vsig)[i] = vsg;
+    if (mhd) {
+        REAL divB = -DB * invrho;
+        REAL divv = -DV * invrho;
+        REAL dbx = -(VBx - bi[0] * DV) * invrho;
+        REAL dby = -(VBy - bi[1] * DV) * invrho;
+        REAL dbz = -(VBz - bi[2] * DV) * invrho;
+        REAL dpsi = (REAL)0;
+        if (clean) {
+            dbx -= rho[i] * PSx;
+            dby -= rho[i] * PSy;
+            dbz -= rho[i] * PSz;
+            REAL psi_i = RA(st->psi)[i];
+            dpsi = -ch * ch * divB - psi_i * csig * ch * P_i.invh - (REAL)0.5 * psi_i * divv;
+        }
+        RA(st->dbdt[0])[i] = dbx;
+        RA(st->dbdt[1])[i] = dby;
+        RA(st->dbdt[2])[i] = dbz;
+        RA(st->dpsidt)[i] = dpsi;
+        RA(st->divb)[i] = divB;
+    }
+    if (st->sc_a) st->sc_a[i] = sa;
+    if (st->sc_du) st->sc_du[i] = sdu;
+    if (st->sc_db) st->sc_db[i] = sdb;
+    if (st->sc_divb) st->sc_divb[i] = sdiv;
+    if (st->sc_dpsi && clean) {
+        double psi_i = (double)RA(st->psi)[i];
+        st->sc_dpsi[i] = (double)ch * (double)ch * sdiv
+                         + fabs(psi_i * (double)csig * (double)ch * (double)P_i.invh)
+                         + 0.5 * fabs(psi_i) * sdvv / (double)rho[i];
+    }
+}
+
+/* Force pass over the 27-supercell stencil. */
 int FN(orc_forces)(const sph_run_config* cfg, int32_t prec, orc_state* st, const uint32_t* cell_start)
 {
     orc_grid g;
@@ -384,181 +602,16 @@ int FN(orc_forces)(const sph_run_config* cfg, int32_t prec, orc_state* st, const
     FN(pair_coords)(&g, prec, st, pc);
     FN(prim)* pr = (FN(prim)*)malloc((size_t)n * sizeof(FN(prim)) + 8);
     FN(prims)(cfg, st, pr);
-    const REAL H2 = (prec == ORC_PREC_F64) ? (REAL)g.H2d : (REAL)g.H2;
-    const int kern = cfg->kernel;
-    const int mhd = cfg->mhd, clean = cfg->mhd && cfg->cleaning;
-    const REAL alpha = (REAL)cfg->alpha_visc, beta = (REAL)cfg->beta_visc;
-    const REAL borve = (mhd && cfg->borve) ? (REAL)1 : (REAL)0;
-    const REAL ch = (REAL)st->c_h;
-    const REAL csig = (REAL)cfg->clean_sigma;
-    const REAL* m = RA(st->mass);
-    const REAL* hh = RA(st->h);
-    const REAL* rho = RA(st->rho);
-    const REAL *vx = RA(st->vel[0]), *vy = RA(st->vel[1]), *vz = RA(st->vel[2]);
-    const REAL *Bx = mhd ? RA(st->B[0]) : NULL, *By = mhd ? RA(st->B[1]) : NULL,
-               *Bz = mhd ? RA(st->B[2]) : NULL;
+    FN(fctx) F;
+    FN(fctx_init)(cfg, st, (prec == ORC_PREC_F64) ? (REAL)g.H2d : (REAL)g.H2, pc, pr, &F);
 
 #pragma omp parallel for schedule(dynamic, 16)
     for (int cell = 0; cell < g.ncell; ++cell) {
-        int c[3];
-        cell_coords(&g, cell, c);
-        int ncc = 0, ccs[27];
-        REAL sh[27][3];
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    int d[3] = {dx, dy, dz};
-                    if (FN(nb_cell)(&g, prec, cfg, c, d, &ccs[ncc], sh[ncc])) {
-                        if (dx == 0 && dy == 0 && dz == 0) ccs[ncc] |= 0x80000000u;
-                        ++ncc;
-                    }
-                }
+        FN(span) sp[27];
+        const int nsp = FN(cell_spans)(&g, prec, cfg, cell_start, cell, sp);
         for (int64_t i = cell_start[cell]; i < (int64_t)cell_start[cell + 1]; ++i) {
             if (!is_target(st->species[i], st->mask[i])) continue;
-            const FN(prim) P_i = pr[i];
-            REAL pi[3] = {pc[0][i], pc[1][i], pc[2][i]};
-            REAL bi[3] = {0, 0, 0};
-            if (mhd) { bi[0] = Bx[i]; bi[1] = By[i]; bi[2] = Bz[i]; }
-            REAL ax = 0, ay = 0, az = 0, DV = 0, AVU = 0, vmax = 0;
-            REAL Ti = 0, Tj = 0, TBx = 0, TBy = 0, TBz = 0, VBx = 0, VBy = 0, VBz = 0, DB = 0;
-            REAL PSx = 0, PSy = 0, PSz = 0;
-            double sa = 0, sdu = 0, sdb = 0, sdiv = 0, sdvv = 0;
-            for (int k = 0; k < ncc; ++k) {
-                int own = (ccs[k] & 0x80000000u) != 0;
-                int cc = ccs[k] & 0x7fffffff;
-                for (int64_t j = cell_start[cc]; j < (int64_t)cell_start[cc + 1]; ++j) {
-                    if (!is_source(st->species[j], st->mask[j])) continue;
-                    REAL d[3];
-                    REAL r2 = FN(geom)(pi, pc, j, sh[k], d);
-                    if (!(r2 < H2)) continue;
-                    if (own && j == i) continue;
-                    const FN(prim) P_j = pr[j];
-                    REAL invR = (REAL)0, rr = (REAL)0;
-                    if (r2 > (REAL)0) {
-                        invR = (REAL)1 / (REAL)sqrt(r2);
-                        rr = r2 * invR;
-                    }
-                    REAL s_i = FN(kgrad)(kern, rr, invR, rr * P_i.invh, P_i.gfac);
-                    REAL s_j = FN(kgrad)(kern, rr, invR, rr * P_j.invh, P_j.gfac);
-                    REAL dvx = vx[i] - vx[j], dvy = vy[i] - vy[j], dvz = vz[i] - vz[j];
-                    REAL vr = dvx * d[0] + dvy * d[1] + dvz * d[2];
-                    REAL ms_i = m[j] * s_i, ms_j = m[j] * s_j;
-                    /* pressure (+ magnetic pressure) gradient, symmetric form */
-                    REAL fiso = P_i.pr * ms_i + P_j.pr * ms_j;
-                    REAL cax = -fiso * d[0], cay = -fiso * d[1], caz = -fiso * d[2];
-                    DV += ms_i * vr;
-                    /* scales use the magnitudes before cancellation: sum_a |dv_a d_a| for v.r (and
-                     * sum_a |B_a d_a| for B.r below), the bound of an FP32 evaluation in any order */
-                    const double vdn = fabs((double)dvx * (double)d[0]) + fabs((double)dvy * (double)d[1])
-                                       + fabs((double)dvz * (double)d[2]);
-                    sdvv += fabs((double)ms_i) * vdn;
-                    sdu += fabs((double)(P_i.ph * ms_i)) * vdn;
-                    /* signal velocity */
-                    REAL w = vr * invR;
-                    REAL vs = P_i.c + P_j.c - beta * (w < (REAL)0 ? w : (REAL)0);
-                    if (vs > vmax) vmax = vs;
-                    /* artificial viscosity */
-                    if (alpha > (REAL)0 && vr < (REAL)0) {
-                        REAL hb = (REAL)0.5 * (hh[i] + hh[j]);
-                        REAL mu = hb * vr / (r2 + (REAL)0.01 * hb * hb);
-                        REAL cb = (REAL)0.5 * (P_i.c + P_j.c);
-                        REAL rb = (REAL)0.5 * (rho[i] + rho[j]);
-                        REAL Pi = (beta * mu - alpha * cb) * mu / rb;
-                        REAL gv = Pi * (REAL)0.5 * (ms_i + ms_j);
-                        cax -= gv * d[0];
-                        cay -= gv * d[1];
-                        caz -= gv * d[2];
-                        AVU += gv * vr;
-                        sdu += fabs((double)gv) * vdn;
-                    }
-                    ax += cax;
-                    ay += cay;
-                    az += caz;
-                    double cnorm = sqrt((double)cax * (double)cax + (double)cay * (double)cay + (double)caz * (double)caz);
-                    if (mhd) {
-                        REAL Bir = bi[0] * d[0] + bi[1] * d[1] + bi[2] * d[2];
-                        REAL Bjr = Bx[j] * d[0] + By[j] * d[1] + Bz[j] * d[2];
-                        REAL ti = Bir * P_i.bdr * ms_i;
-                        REAL tj = Bjr * P_j.bdr * ms_j;
-                        Ti += ti;
-                        Tj += tj;
-                        TBx += tj * Bx[j];
-                        TBy += tj * By[j];
-                        TBz += tj * Bz[j];
-                        REAL wv = ms_i * Bir;
-                        VBx += wv * dvx;
-                        VBy += wv * dvy;
-                        VBz += wv * dvz;
-                        REAL db = ms_i * (Bir - Bjr);
-                        DB += db;
-                        double bn_i = sqrt((double)bi[0] * (double)bi[0] + (double)bi[1] * (double)bi[1] + (double)bi[2] * (double)bi[2]);
-                        double bn_j = sqrt((double)Bx[j] * (double)Bx[j] + (double)By[j] * (double)By[j] + (double)Bz[j] * (double)Bz[j]);
-                        const double bdi = fabs((double)bi[0] * (double)d[0]) + fabs((double)bi[1] * (double)d[1])
-                                           + fabs((double)bi[2] * (double)d[2]);
-                        const double bdj = fabs((double)Bx[j] * (double)d[0]) + fabs((double)By[j] * (double)d[1])
-                                           + fabs((double)Bz[j] * (double)d[2]);
-                        const double ati = fabs((double)(P_i.bdr * ms_i)) * bdi;
-                        const double atj = fabs((double)(P_j.bdr * ms_j)) * bdj;
-                        cnorm += ati * bn_i + atj * bn_j + (double)borve * (ati + atj) * bn_i;
-                        double dvn = sqrt((double)dvx * (double)dvx + (double)dvy * (double)dvy + (double)dvz * (double)dvz);
-                        sdb += (fabs((double)ms_i) * bdi * dvn + bn_i * fabs((double)ms_i) * vdn) / (double)rho[i];
-                        sdiv += fabs((double)ms_i) * (bdi + bdj) / (double)rho[i];
-                        if (clean) {
-                            REAL fps = P_i.psr * ms_i + P_j.psr * ms_j;
-                            PSx += fps * d[0];
-                            PSy += fps * d[1];
-                            PSz += fps * d[2];
-                            double dn = sqrt((double)d[0] * (double)d[0] + (double)d[1] * (double)d[1] + (double)d[2] * (double)d[2]);
-                            sdb += (double)rho[i] * (fabs((double)(P_i.psr * ms_i)) + fabs((double)(P_j.psr * ms_j))) * dn;
-                        }
-                    }
-                    sa += cnorm;
-                }
-            }
-            REAL invrho = (REAL)1 / rho[i];
-            if (mhd) {
-                REAL tcoef = Ti - borve * (Ti + Tj);
-                ax = ax + TBx + bi[0] * tcoef;
-                ay = ay + TBy + bi[1] * tcoef;
-                az = az + TBz + bi[2] * tcoef;
-            }
-            RA(st->dvdt[0])[i] = ax;
-            RA(st->dvdt[1])[i] = ay;
-            RA(st->dvdt[2])[i] = az;
-            RA(st->dudt)[i] = P_i.ph * DV + (REAL)0.5 * AVU;
-            REAL vsg = (REAL)2 * P_i.c;
-            if (vmax > vsg) vsg = vmax;
-            RA(st->vsig)[i] = vsg;
-            if (mhd) {
-                REAL divB = -DB * invrho;
-                REAL divv = -DV * invrho;
-                REAL dbx = -(VBx - bi[0] * DV) * invrho;
-                REAL dby = -(VBy - bi[1] * DV) * invrho;
-                REAL dbz = -(VBz - bi[2] * DV) * invrho;
-                REAL dpsi = (REAL)0;
-                if (clean) {
-                    dbx -= rho[i] * PSx;
-                    dby -= rho[i] * PSy;
-                    dbz -= rho[i] * PSz;
-                    REAL psi_i = RA(st->psi)[i];
-                    dpsi = -ch * ch * divB - psi_i * csig * ch * P_i.invh - (REAL)0.5 * psi_i * divv;
-                }
-                RA(st->dbdt[0])[i] = dbx;
-                RA(st->dbdt[1])[i] = dby;
-                RA(st->dbdt[2])[i] = dbz;
-                RA(st->dpsidt)[i] = dpsi;
-                RA(st->divb)[i] = divB;
-            }
-            if (st->sc_a) st->sc_a[i] = sa;
-            if (st->sc_du) st->sc_du[i] = sdu;
-            if (st->sc_db) st->sc_db[i] = sdb;
-            if (st->sc_divb) st->sc_divb[i] = sdiv;
-            if (st->sc_dpsi && clean) {
-                double psi_i = (double)RA(st->psi)[i];
-                st->sc_dpsi[i] = (double)ch * (double)ch * sdiv
-                                 + fabs(psi_i * (double)csig * (double)ch * (double)P_i.invh)
-                                 + 0.5 * fabs(psi_i) * sdvv / (double)rho[i];
-            }
+            FN(force_one)(&F, st, i, sp, nsp);
         }
     }
     free(pr);
@@ -566,8 +619,9 @@ int FN(orc_forces)(const sph_run_config* cfg, int32_t prec, orc_state* st, const
     return 0;
 }
 
-/* Drift: PushVelocity (PushVelocity.hpp:35-42) for Movable species, then periodic wrap. */
-int FN(orc_drift)(const sph_run_config* cfg, int32_t prec, orc_state* st, double dt)
+/* Drift: PushVelocity (PushVelocity.hpp:35-42) for Movable species, then the periodic wrap (skipped
+ * when wrap = 0: the region traversal keeps positions continuous, as the reference does). */
+static int FN(drift_impl)(const sph_run_config* cfg, int32_t prec, orc_state* st, double dt, int wrap)
 {
     orc_grid g;
     if (orc_make_grid(cfg, &g)) return -1;
@@ -582,7 +636,7 @@ int FN(orc_drift)(const sph_run_config* cfg, int32_t prec, orc_state* st, double
                 float dts = (float)dt;
                 float vv = (float)v;
                 x = x + vv * dts;
-                if (cfg->periodic[a]) {
+                if (wrap && cfg->periodic[a]) {
                     float hi = g.lo32[a] + g.L32[a];
                     if (x >= hi) x = x - g.L32[a];
                     else if (x < g.lo32[a]) x = x + g.L32[a];
@@ -591,7 +645,7 @@ int FN(orc_drift)(const sph_run_config* cfg, int32_t prec, orc_state* st, double
             } else {
                 double x = st->pos[a][i];
                 x = x + (double)v * dt;
-                if (cfg->periodic[a]) {
+                if (wrap && cfg->periodic[a]) {
                     double hi = g.lo[a] + g.L[a];
                     if (x >= hi) x = x - g.L[a];
                     else if (x < g.lo[a]) x = x + g.L[a];
@@ -601,6 +655,11 @@ int FN(orc_drift)(const sph_run_config* cfg, int32_t prec, orc_state* st, double
         }
     }
     return 0;
+}
+
+int FN(orc_drift)(const sph_run_config* cfg, int32_t prec, orc_state* st, double dt)
+{
+    return FN(drift_impl)(cfg, prec, st, dt, 1);
 }
 
 /* CFL dt of one target (new; Monaghan 1997 signal velocity, Courant + force criteria). */
@@ -764,6 +823,226 @@ int FN(orc_brute_counts)(const sph_run_config* cfg, int32_t prec, orc_state* st)
     }
     for (int a = 0; a < 3; ++a) free(pc[a]);
     return 0;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Region traversal: the reference's own neighbour algorithm, restated for the CPU baseline. The
+ * state is laid out by orc_regions_layout: regions (blocks of supercells at set-up) keep their
+ * particles for the whole run, in 64-slot frames (ParticleRegion.hpp:37-62, numFrameSlots = 64 in
+ * param/memory.param:35), unused slots dead (mask 0). Each step:
+ *   - refits each region's AABB to its live sources (UpdateRegionBounds, RegionBoundsUpdate.hpp:39-165);
+ *   - brute-force region search: every source region whose AABB intersects the target region's AABB
+ *     expanded by H, counted then written as CSR (FindNeighbourRegionsFunctor,
+ *     NeighbourRegions.hpp:55-135; calculateNeighbourEntrySender :140-230);
+ *   - density and force passes frame by frame: every target slot against every slot of every frame
+ *     of every neighbour region, dead slots tested and rejected (interactWithNeighbourFrame,
+ *     ParticleParticleInteraction.hpp:219-314), one OpenMP iteration per target frame.
+ * Periodic axes (which the reference lacks) test each source AABB under the images -L, 0, +L and
+ * keep one CSR entry per overlapping image; positions are not wrapped, so a region stays compact.
+ * The expansion is H widened by 1e-6 relative, so FP32 rounding of the bounds never drops a pair
+ * within H. No precision mode but absolute coordinates (plain FP32, FP64).
+ * --------------------------------------------------------------------------------------------- */
+
+/* Images (bit w+1 for shift w L, w in -1..1) under which source interval [s0, s1] meets [t0, t1];
+ * the inclusive test of AABB.hpp:116-119. */
+static inline unsigned FN(axis_images)(REAL t0, REAL t1, REAL s0, REAL s1, REAL L, int periodic)
+{
+    unsigned m = !(t0 > s1 || t1 < s0) ? 2u : 0u;
+    if (periodic) {
+        if (!(t0 > s1 - L || t1 < s0 - L)) m |= 1u;
+        if (!(t0 > s1 + L || t1 < s0 + L)) m |= 4u;
+    }
+    return m;
+}
+
+static const int FN(pop3)[8] = {0, 1, 1, 2, 1, 2, 2, 3};
+
+/* The CSR row of region r as spans: each neighbour region's slots under its image shift. */
+static int FN(region_spans)(const int64_t* region_start, const int64_t* off, const int32_t* ent, const uint8_t* img,
+                            int32_t r, const REAL Lr[3], FN(span)* sp)
+{
+    int nsp = 0;
+    for (int64_t e = off[r]; e < off[r + 1]; ++e) {
+        const int q = ent[e], code = img[e];
+        const int w[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+        sp[nsp].b = region_start[q];
+        sp[nsp].e = region_start[q + 1];
+        sp[nsp].own = (q == r && code == 13);
+        for (int a = 0; a < 3; ++a) sp[nsp].sh[a] = (REAL)w[a] * Lr[a];
+        ++nsp;
+    }
+    return nsp;
+}
+
+int FN(orc_regions_step)(const sph_run_config* cfg, int32_t prec, orc_state* st, int32_t n_regions,
+                         const int64_t* region_start, int32_t n_steps, int64_t* csr_entries, sph_conserved* out)
+{
+    if (prec == ORC_PREC_MIXED || n_regions < 1 || n_steps < 0) return -1;
+    orc_grid g;
+    if (orc_make_grid(cfg, &g)) return -1;
+    const int64_t n = st->n;
+    if (n % 64 != 0 || region_start[0] != 0 || region_start[n_regions] != n) return -1;
+    for (int32_t r = 0; r < n_regions; ++r)
+        if (region_start[r + 1] <= region_start[r] || region_start[r + 1] % 64 != 0) return -1;
+    const int64_t nframes = n / 64;
+    const int32_t R = n_regions;
+    const REAL H = (prec == ORC_PREC_F64) ? (REAL)g.Hd : (REAL)g.H;
+    const REAL H2 = (prec == ORC_PREC_F64) ? (REAL)g.H2d : (REAL)g.H2;
+    const REAL Hs = H + H * (REAL)1e-6;
+    REAL Lr[3];
+    for (int a = 0; a < 3; ++a) Lr[a] = (prec == ORC_PREC_F64) ? (REAL)g.L[a] : (REAL)g.L32[a];
+    const REAL f0 = FN(kshape)(cfg->kernel, (REAL)0);
+
+    int32_t* frame_region = (int32_t*)malloc((size_t)nframes * 4 + 4);
+    int64_t* off = (int64_t*)malloc(((size_t)R + 1) * 8);
+    REAL* bb = (REAL*)malloc((size_t)R * 6 * sizeof(REAL));
+    REAL* pc[3];
+    for (int a = 0; a < 3; ++a) pc[a] = (REAL*)malloc((size_t)n * sizeof(REAL) + 8);
+    FN(prim)* pr = (FN(prim)*)malloc((size_t)n * sizeof(FN(prim)) + 8);
+    if (!frame_region || !off || !bb || !pc[0] || !pc[1] || !pc[2] || !pr) return -2;
+    for (int32_t r = 0; r < R; ++r)
+        for (int64_t f = region_start[r] / 64; f < region_start[r + 1] / 64; ++f) frame_region[f] = r;
+    REAL *lo[3], *hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = bb + (size_t)a * R;
+        hi[a] = bb + (size_t)(3 + a) * R;
+    }
+
+    int rc = 0;
+    sph_conserved c;
+    memset(&c, 0, sizeof(c));
+    const int prime = (n_steps == 0);
+    for (int s = 0; s < (prime ? 1 : n_steps) && !rc; ++s) {
+        double dt = cfg->dt_fixed > 0 ? cfg->dt_fixed : st->dt;
+        if (!prime && !(dt > 0)) { rc = -4; break; }
+        const int advance = !prime && dt < HUGE_VAL;
+        if (advance) rc = FN(drift_impl)(cfg, prec, st, dt, 0);
+        if (rc) break;
+        FN(pair_coords)(&g, prec, st, pc);
+
+        /* AABB refit over live sources; a region without any keeps an empty box */
+#pragma omp parallel for schedule(static)
+        for (int32_t r = 0; r < R; ++r) {
+            REAL l[3], h[3];
+            for (int a = 0; a < 3; ++a) {
+                l[a] = (REAL)HUGE_VAL;
+                h[a] = (REAL)-HUGE_VAL;
+            }
+            for (int64_t i = region_start[r]; i < region_start[r + 1]; ++i) {
+                if (!is_source(st->species[i], st->mask[i])) continue;
+                for (int a = 0; a < 3; ++a) {
+                    if (pc[a][i] < l[a]) l[a] = pc[a][i];
+                    if (pc[a][i] > h[a]) h[a] = pc[a][i];
+                }
+            }
+            for (int a = 0; a < 3; ++a) {
+                lo[a][r] = l[a];
+                hi[a][r] = h[a];
+            }
+        }
+
+        /* region search, Count pass */
+        int64_t maxrow = 0;
+        off[0] = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(max : maxrow)
+        for (int32_t r = 0; r < R; ++r) {
+            REAL t0[3], t1[3];
+            for (int a = 0; a < 3; ++a) {
+                t0[a] = lo[a][r] - Hs;
+                t1[a] = hi[a][r] + Hs;
+            }
+            int64_t cnt = 0;
+            for (int32_t q = 0; q < R; ++q) {
+                unsigned mx = FN(axis_images)(t0[0], t1[0], lo[0][q], hi[0][q], Lr[0], cfg->periodic[0]);
+                if (!mx) continue;
+                unsigned my = FN(axis_images)(t0[1], t1[1], lo[1][q], hi[1][q], Lr[1], cfg->periodic[1]);
+                if (!my) continue;
+                unsigned mz = FN(axis_images)(t0[2], t1[2], lo[2][q], hi[2][q], Lr[2], cfg->periodic[2]);
+                cnt += FN(pop3)[mx] * FN(pop3)[my] * FN(pop3)[mz];
+            }
+            off[r + 1] = cnt;
+            if (cnt > maxrow) maxrow = cnt;
+        }
+        for (int32_t r = 0; r < R; ++r) off[r + 1] += off[r];
+        const int64_t total = off[R];
+        if (csr_entries) *csr_entries = total;
+        int32_t* ent = (int32_t*)malloc((size_t)total * 4 + 4);
+        uint8_t* img = (uint8_t*)malloc((size_t)total + 1);
+        if (!ent || !img) { rc = -2; break; }
+
+        /* region search, Write pass: sources in index order, images in (z, y, x) order */
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int32_t r = 0; r < R; ++r) {
+            REAL t0[3], t1[3];
+            for (int a = 0; a < 3; ++a) {
+                t0[a] = lo[a][r] - Hs;
+                t1[a] = hi[a][r] + Hs;
+            }
+            int64_t w = off[r];
+            for (int32_t q = 0; q < R; ++q) {
+                unsigned mx = FN(axis_images)(t0[0], t1[0], lo[0][q], hi[0][q], Lr[0], cfg->periodic[0]);
+                if (!mx) continue;
+                unsigned my = FN(axis_images)(t0[1], t1[1], lo[1][q], hi[1][q], Lr[1], cfg->periodic[1]);
+                if (!my) continue;
+                unsigned mz = FN(axis_images)(t0[2], t1[2], lo[2][q], hi[2][q], Lr[2], cfg->periodic[2]);
+                for (int wz = 0; wz < 3; ++wz)
+                    for (int wy = 0; wy < 3; ++wy)
+                        for (int wx = 0; wx < 3; ++wx)
+                            if (((mx >> wx) & 1u) && ((my >> wy) & 1u) && ((mz >> wz) & 1u)) {
+                                ent[w] = q;
+                                img[w] = (uint8_t)(wx + 3 * wy + 9 * wz);
+                                ++w;
+                            }
+            }
+        }
+
+        /* density pass, one target frame per iteration */
+#pragma omp parallel
+        {
+            FN(span)* sp = (FN(span)*)malloc((size_t)(maxrow + 1) * sizeof(FN(span)));
+#pragma omp for schedule(dynamic, 16)
+            for (int64_t f = 0; f < nframes; ++f) {
+                const int nsp = FN(region_spans)(region_start, off, ent, img, frame_region[f], Lr, sp);
+                for (int64_t i = f * 64; i < f * 64 + 64; ++i) {
+                    if (!is_target(st->species[i], st->mask[i])) {
+                        if (st->nnb) st->nnb[i] = 0;
+                        continue;
+                    }
+                    FN(density_one)(cfg, st, pc, H2, f0, i, sp, nsp);
+                }
+            }
+            free(sp);
+        }
+        st->c_h = FN(clean_speed)(cfg, st);
+
+        /* force pass, same traversal */
+        FN(prims)(cfg, st, pr);
+        FN(fctx) F;
+        FN(fctx_init)(cfg, st, H2, pc, pr, &F);
+#pragma omp parallel
+        {
+            FN(span)* sp = (FN(span)*)malloc((size_t)(maxrow + 1) * sizeof(FN(span)));
+#pragma omp for schedule(dynamic, 16)
+            for (int64_t f = 0; f < nframes; ++f) {
+                const int nsp = FN(region_spans)(region_start, off, ent, img, frame_region[f], Lr, sp);
+                for (int64_t i = f * 64; i < f * 64 + 64; ++i)
+                    if (is_target(st->species[i], st->mask[i])) FN(force_one)(&F, st, i, sp, nsp);
+            }
+            free(sp);
+        }
+        free(ent);
+        free(img);
+
+        rc = advance ? FN(orc_kick)(cfg, prec, st, dt, &c) : FN(orc_reduce)(cfg, prec, st, &c);
+        if (!rc) st->dt = c.dt_next;
+    }
+    if (out) *out = c;
+    free(frame_region);
+    free(off);
+    free(bb);
+    for (int a = 0; a < 3; ++a) free(pc[a]);
+    free(pr);
+    return rc;
 }
 
 #undef FN

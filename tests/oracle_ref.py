@@ -74,10 +74,15 @@ def lib():
             getattr(L, f"orc_prime_{s}").argtypes = [cfgp, C.c_int32, stp, C.POINTER(Conserved)]
             getattr(L, f"orc_step_{s}").argtypes = [cfgp, C.c_int32, stp, C.c_int32, C.POINTER(Conserved)]
             getattr(L, f"orc_brute_counts_{s}").argtypes = [cfgp, C.c_int32, stp]
+            getattr(L, f"orc_regions_step_{s}").argtypes = [cfgp, C.c_int32, stp, C.c_int32, C.POINTER(C.c_int64),
+                                                            C.c_int32, C.POINTER(C.c_int64), C.POINTER(Conserved)]
             for nm in ("w", "dwdr"):
                 f = getattr(L, f"orc_kernel_{nm}_{s}")
                 f.argtypes = [C.c_int32, C.c_double, C.c_double]
                 f.restype = C.c_double
+        L.orc_regions_layout.argtypes = [cfgp, C.c_int32, C.c_int64, _dp * 3, C.POINTER(C.c_uint64), C.c_int32,
+                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int32,
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -242,3 +247,62 @@ class OracleRun:
     def by_id(self):
         """Permutation that orders the current arrays by particle id."""
         return np.argsort(self.id, kind="stable")
+
+
+def regions_layout(cfg: RunConfig, prec: str, ps: ParticleSet, block: int = 1):
+    """Slots of the region traversal: (slot per particle, region_start, number of slots)."""
+    g = grid(cfg)
+    nb = [(g.n[a] + block - 1) // block for a in range(3)]
+    cap = nb[0] * nb[1] * nb[2]
+    pos = np.ascontiguousarray(ps.pos, dtype=np.float64)
+    ids = np.ascontiguousarray(ps.id, dtype=np.uint64)
+    slot = np.zeros(ps.n, dtype=np.int64)
+    rs = np.zeros(cap + 1, dtype=np.int64)
+    nr, ns = C.c_int32(0), C.c_int64(0)
+    arr = (_dp * 3)(*[pos[a].ctypes.data_as(_dp) for a in range(3)])
+    rc = lib().orc_regions_layout(C.byref(cfg), PREC[prec], ps.n, arr, ids.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                  block, slot.ctypes.data_as(C.POINTER(C.c_int64)),
+                                  rs.ctypes.data_as(C.POINTER(C.c_int64)), cap, C.byref(nr), C.byref(ns))
+    assert rc == 0, f"orc_regions_layout failed: {rc}"
+    return slot, rs[: nr.value + 1].copy(), ns.value
+
+
+class RegionRun(OracleRun):
+    """One run stepped by the reference's region/frame traversal (orc_regions_step): particles stay
+    in the 64-slot frames of their set-up region; unused slots are dead. Absolute coordinates only
+    ('plain' FP32 or 'f64'). Fields are held per slot; gather() returns them in input order."""
+
+    def __init__(self, cfg: RunConfig, ps: ParticleSet, prec: str = "plain", block: int = 1,
+                 scales: bool = False):
+        slot, rs, ns = regions_layout(cfg, prec, ps, block)
+        padded = ParticleSet(ns)
+        padded.id[:] = 0
+        padded.mask[:] = 0
+        for k, v in ps.__dict__.items():
+            if isinstance(v, np.ndarray):
+                getattr(padded, k)[..., slot] = v
+        super().__init__(cfg, padded, prec, scales)
+        self.slot = slot
+        self.region_start = rs
+        self.n_regions = rs.size - 1
+        self.block = block
+        self.csr_entries = 0
+
+    def _regions(self, n_steps: int) -> dict:
+        out = Conserved()
+        ce = C.c_int64(0)
+        fn = getattr(lib(), f"orc_regions_step_{self.sfx}")
+        rc = fn(C.byref(self.cfg), self.pcode, C.byref(self.st), self.n_regions,
+                self.region_start.ctypes.data_as(C.POINTER(C.c_int64)), int(n_steps), C.byref(ce), C.byref(out))
+        assert rc == 0, f"orc_regions_step failed: {rc}"
+        self.csr_entries = ce.value
+        return out.as_dict()
+
+    def prime(self) -> dict:
+        return self._regions(0)
+
+    def step(self, n: int) -> dict:
+        return self._regions(n)
+
+    def gather(self, name: str) -> np.ndarray:
+        return getattr(self, name)[..., self.slot]

@@ -1,0 +1,164 @@
+"""The reference's region/frame traversal (oracle/sph_oracle_impl.h, orc_regions_step; CPU only).
+
+This is the CPU baseline's reference-structured algorithm. Regions are blocks of supercells fixed at
+set-up, stored in 64-slot frames. Each step refits their AABBs, searches all region pairs by brute
+force (NeighbourRegions.hpp:55-135) and sweeps every slot of every neighbour frame
+(ParticleParticleInteraction.hpp:219-314). It shares the per-target physics with the binned
+stencil, so it must find the same neighbours. The checks:
+  * the layout: one slot per particle, frames aligned to 64, one supercell block per region, id
+    order inside a region;
+  * the CSR against a numpy brute-force intersection of the refitted AABBs (the inclusive test of
+    AABB.hpp:116-119 under the periodic images);
+  * neighbour counts bit-exact against the binned stencil and the O(N^2) brute force, with dead
+    particles, walls outside a non-periodic box and both precisions with absolute coordinates;
+    densities and derivatives within the FP32 summation-order bound of their scales;
+  * three steps against the binned traversal: the same conserved sums and positions modulo the box.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle_ref as O
+from spearhed_amd import ic
+
+
+def _cases():
+    cfg_t, ps_t = ic.mhd_turbulence(n=12, seed=11)
+    cfg_k, ps_k = ic.kelvin_helmholtz(2500, 12)
+    cfg_s, ps_s = ic.sod_tube(seed=13, nx_left=24, cross=6)
+    ps_d = ps_t.copy()
+    ps_d.mask[np.random.default_rng(14).uniform(size=ps_d.n) < 0.1] = 0
+    return {"turbulence": (cfg_t, ps_t), "kh": (cfg_k, ps_k), "sod_walls": (cfg_s, ps_s),
+            "turbulence_dead": (cfg_t, ps_d)}
+
+
+CASES = _cases()
+
+
+def _binned_in_input_order(o: O.OracleRun, ps, name):
+    """Binned oracle field reordered to the input particle order (matched by id)."""
+    pos_of_id = {int(v): k for k, v in enumerate(o.id)}
+    idx = np.array([pos_of_id[int(v)] for v in ps.id])
+    return getattr(o, name)[..., idx]
+
+
+def test_layout(oracle_lib):
+    cfg, ps = CASES["kh"]
+    for block in (1, 2):
+        slot, rs, ns = O.regions_layout(cfg, "plain", ps, block)
+        assert np.unique(slot).size == ps.n and slot.min() >= 0 and slot.max() < ns
+        assert np.all(rs % 64 == 0) and np.all(np.diff(rs) > 0) and rs[-1] == ns
+        g = O.grid(cfg)
+        key = O.bin_keys(cfg, "plain", ps.pos)
+        cx, cy, cz = key % g.n[0], (key // g.n[0]) % g.n[1], key // (g.n[0] * g.n[1])
+        blk = (cx // block) + 1000 * (cy // block) + 1000000 * (cz // block)
+        region = np.searchsorted(rs, slot, side="right") - 1
+        for r in np.unique(region):
+            sel = np.flatnonzero(region == r)
+            assert np.unique(blk[sel]).size == 1
+            order = np.argsort(slot[sel])
+            assert np.all(np.diff(ps.id[sel][order].astype(np.int64)) > 0)
+            assert slot[sel].min() == rs[r] and slot[sel].max() == rs[r] + sel.size - 1
+
+
+def _numpy_csr_entries(cfg, run: O.RegionRun) -> int:
+    g = O.grid(cfg)
+    H = np.float32(g.H)
+    Hs = np.float32(H + H * np.float32(1e-6))
+    R = run.n_regions
+    lo = np.full((3, R), np.inf, np.float32)
+    hi = np.full((3, R), -np.inf, np.float32)
+    live = (run.mask == 1) & (run.species <= 1)
+    for r in range(R):
+        sl = slice(run.region_start[r], run.region_start[r + 1])
+        sel = live[sl]
+        if sel.any():
+            p = run.pos[:, sl][:, sel].astype(np.float32)
+            lo[:, r] = p.min(axis=1)
+            hi[:, r] = p.max(axis=1)
+    total = 0
+    t0 = lo - Hs
+    t1 = hi + Hs
+    for r in range(R):
+        cnt = np.ones(R, np.int64)
+        for a in range(3):
+            L = np.float32(g.L32[a])
+            shifts = (np.float32(-L), np.float32(0), L) if cfg.periodic[a] else (np.float32(0),)
+            n_img = np.zeros(R, np.int64)
+            for s in shifts:
+                n_img += ~((t0[a, r] > hi[a] + s) | (t1[a, r] < lo[a] + s))
+            cnt *= n_img
+        total += int(cnt.sum())
+    return total
+
+
+@pytest.mark.parametrize("case", ["kh", "sod_walls"])
+def test_search_matches_numpy_brute_force(oracle_lib, case):
+    cfg, ps = CASES[case]
+    run = O.RegionRun(cfg, ps, "plain", block=1)
+    run.prime()
+    assert run.csr_entries == _numpy_csr_entries(cfg, run)
+    run.step(2)  # after drifting (positions unwrapped), the refit boxes still give the same CSR
+    assert run.csr_entries == _numpy_csr_entries(cfg, run)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("prec,block", [("plain", 1), ("plain", 2), ("f64", 1)])
+def test_prime_matches_binned(oracle_lib, case, prec, block):
+    cfg, ps = CASES[case]
+    reg = O.RegionRun(cfg, ps, prec, block=block, scales=True)
+    cr = reg.prime()
+    ref = O.OracleRun(cfg, ps, prec, scales=True)
+    cb = ref.prime()
+    t = (ps.mask == 1) & (ps.species == 0)
+    assert np.array_equal(reg.gather("nnb"), _binned_in_input_order(ref, ps, "nnb"))
+    brute = O.OracleRun(cfg, ps, prec)
+    assert np.array_equal(reg.gather("nnb"), brute.brute_counts())  # input order: no resort done
+    tol = 1e-6 if prec == "plain" else 1e-13
+    for f, sc in (("rho", "sc_rho"), ("dvdt", "sc_a"), ("dudt", "sc_du")) + \
+                 ((("dbdt", "sc_db"), ("divb", "sc_divb")) if cfg.mhd else ()):
+        a = reg.gather(f).astype(np.float64)
+        b = _binned_in_input_order(ref, ps, f).astype(np.float64)
+        s = _binned_in_input_order(ref, ps, sc)
+        err = np.abs(a - b)[..., t] / (s[t] + 1e-300)
+        assert err.max() <= 8 * tol, (f, err.max())
+    assert cr["dt_next"] == pytest.approx(cb["dt_next"], rel=10 * tol)
+    assert cr["mass"] == cb["mass"]
+    assert cr["c_h"] == pytest.approx(cb["c_h"], rel=10 * tol)
+
+
+@pytest.mark.parametrize("case", ["turbulence", "sod_walls"])
+def test_steps_track_binned(oracle_lib, case):
+    cfg, ps = CASES[case]
+    reg = O.RegionRun(cfg, ps, "plain", block=2)
+    reg.prime()
+    ref = O.OracleRun(cfg, ps, "plain")
+    ref.prime()
+    cr = reg.step(3)
+    cb = ref.step(3)
+    assert cr["time"] == pytest.approx(cb["time"], rel=1e-5)
+    assert cr["mass"] == cb["mass"]
+    assert cr["energy"] == pytest.approx(cb["energy"], rel=1e-5)
+    pabs = float(np.sum(ref.mass.astype(np.float64) * np.abs(ref.vel).sum(axis=0)))  # sum m |v| after the steps
+    for a in range(3):
+        assert abs(cr["momentum"][a] - cb["momentum"][a]) <= 1e-5 * pabs
+    # positions: the binned run wraps into the box, the region run keeps them continuous
+    L = np.array(cfg.box_len)[:, None]
+    lo = np.array(cfg.box_lo)[:, None]
+    pr = reg.gather("pos")
+    pb = _binned_in_input_order(ref, ps, "pos")
+    per = np.array(cfg.periodic, bool)
+    d = pr - pb
+    d[per] -= np.round(d[per] / L[per]) * L[per]
+    assert np.abs(d).max() <= 1e-5 * L.max()
+    assert np.array_equal(reg.gather("nnb"), _binned_in_input_order(ref, ps, "nnb")) or \
+        np.mean(reg.gather("nnb") != _binned_in_input_order(ref, ps, "nnb")) < 1e-3
+
+
+def test_rejects_supercell_relative_coordinates(oracle_lib):
+    cfg, ps = CASES["kh"]
+    run = O.RegionRun(cfg, ps, "plain")
+    with pytest.raises(AssertionError, match="orc_regions_step failed: -1"):
+        run.pcode = O.PREC["mixed"]
+        run.prime()
