@@ -212,6 +212,14 @@ def fp64_error(cfg, ps, g):
     return out
 
 
+_T0 = time.perf_counter()
+
+
+def _progress(phase: str):
+    """One line per finished phase on stderr (the JSON result stays the only stdout line)."""
+    print(f"[bench {time.perf_counter() - _T0:6.1f} s] {phase} done", file=sys.stderr, flush=True)
+
+
 def _time_steps(run, budget_s: float):
     steps = 0
     t0 = time.perf_counter()
@@ -333,13 +341,18 @@ def main():
         },
         "conserved": m["cons"],
     }
+    _progress(f"main workload: {value:.4g} particle-steps/s")
     if not args.no_ensemble:
         out["ensemble"] = run_ensemble(args)
+        _progress("ensemble")
     if not args.no_serve:
         out["serving"] = run_serving(args)
+        _progress("serving")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_reference_structured(m["cfg"], m["ps"], args.cpu_seconds, args.cpu_region_block)
+        _progress("CPU reference-structured")
         out["cpu_baseline_binned"] = cpu_baseline(m["cfg"], m["ps"], args.cpu_seconds)
+        _progress("CPU binned")
         out["fp64_oracle_error"] = fp64_error(m["cfg"], m["ps"], m["primed"])
         out["vs_cpu"] = value / out["cpu_baseline"]["value"]
         out["vs_cpu_binned"] = value / out["cpu_baseline_binned"]["value"]
