@@ -857,6 +857,65 @@ static inline unsigned FN(axis_images)(REAL t0, REAL t1, REAL s0, REAL s1, REAL 
 
 static const int FN(pop3)[8] = {0, 1, 1, 2, 1, 2, 2, 3};
 
+/* Brute-force region search (FindNeighbourRegionsFunctor, NeighbourRegions.hpp:55-135): for each
+ * target box, every source box that meets it expanded by `expand`, once per periodic image that
+ * does. Boxes are SoA: lo x, y, z then hi x, y, z, each an array of the box count.
+ * Count pass (ent == NULL): off[r + 1] = entries of target r, then scanned with off[0] = 0 (the
+ * Count kernel and inclusiveScanHost, NeighbourRegions.hpp:167-193). Write pass: target r's entries
+ * from off[r], sources in index order, images in (z, y, x) order (the reference orders a row by
+ * block atomics, so there only the set of a row is fixed). Returns the entry count. */
+static int64_t FN(region_search)(int32_t nt, const REAL* tb, int32_t ns, const REAL* sb, REAL expand,
+                                 const REAL L[3], const int32_t periodic[3], int64_t* off, int32_t* ent,
+                                 uint8_t* img)
+{
+    const REAL* slo[3] = {sb, sb + ns, sb + 2 * (size_t)ns};
+    const REAL* shi[3] = {sb + 3 * (size_t)ns, sb + 4 * (size_t)ns, sb + 5 * (size_t)ns};
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int32_t r = 0; r < nt; ++r) {
+        REAL t0[3], t1[3];
+        for (int a = 0; a < 3; ++a) {
+            t0[a] = tb[(size_t)a * nt + r] - expand;
+            t1[a] = tb[(size_t)(3 + a) * nt + r] + expand;
+        }
+        int64_t w = ent ? off[r] : 0;
+        for (int32_t q = 0; q < ns; ++q) {
+            unsigned mx = FN(axis_images)(t0[0], t1[0], slo[0][q], shi[0][q], L[0], periodic[0]);
+            if (!mx) continue;
+            unsigned my = FN(axis_images)(t0[1], t1[1], slo[1][q], shi[1][q], L[1], periodic[1]);
+            if (!my) continue;
+            unsigned mz = FN(axis_images)(t0[2], t1[2], slo[2][q], shi[2][q], L[2], periodic[2]);
+            if (!ent) {
+                w += FN(pop3)[mx] * FN(pop3)[my] * FN(pop3)[mz];
+                continue;
+            }
+            for (int wz = 0; wz < 3; ++wz)
+                for (int wy = 0; wy < 3; ++wy)
+                    for (int wx = 0; wx < 3; ++wx)
+                        if (((mx >> wx) & 1u) && ((my >> wy) & 1u) && ((mz >> wz) & 1u)) {
+                            ent[w] = q;
+                            img[w] = (uint8_t)(wx + 3 * wy + 9 * wz);
+                            ++w;
+                        }
+        }
+        if (!ent) off[r + 1] = w;
+    }
+    if (!ent) {
+        off[0] = 0;
+        for (int32_t r = 0; r < nt; ++r) off[r + 1] += off[r];
+    }
+    return off[nt];
+}
+
+/* The search on caller-given boxes (REAL arrays, 6 per box as above). */
+int64_t FN(orc_region_search)(int32_t nt, const void* tbox, int32_t ns, const void* sbox, double expand,
+                              const double L[3], const int32_t periodic[3], int64_t* off, int32_t* ent, uint8_t* img)
+{
+    REAL Lr[3];
+    for (int a = 0; a < 3; ++a) Lr[a] = (REAL)L[a];
+    return FN(region_search)(nt, (const REAL*)tbox, ns, (const REAL*)sbox, (REAL)expand, Lr, periodic, off, ent,
+                             img);
+}
+
 /* The CSR row of region r as spans: each neighbour region's slots under its image shift. */
 static int FN(region_spans)(const int64_t* region_start, const int64_t* off, const int32_t* ent, const uint8_t* img,
                             int32_t r, const REAL Lr[3], FN(span)* sp)
@@ -941,60 +1000,21 @@ int FN(orc_regions_step)(const sph_run_config* cfg, int32_t prec, orc_state* st,
             }
         }
 
-        /* region search, Count pass */
+        /* region search: Count, scan, Write (every region is both a target and a source) */
+        const int64_t total = FN(region_search)(R, bb, R, bb, Hs, Lr, cfg->periodic, off, NULL, NULL);
         int64_t maxrow = 0;
-        off[0] = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(max : maxrow)
-        for (int32_t r = 0; r < R; ++r) {
-            REAL t0[3], t1[3];
-            for (int a = 0; a < 3; ++a) {
-                t0[a] = lo[a][r] - Hs;
-                t1[a] = hi[a][r] + Hs;
-            }
-            int64_t cnt = 0;
-            for (int32_t q = 0; q < R; ++q) {
-                unsigned mx = FN(axis_images)(t0[0], t1[0], lo[0][q], hi[0][q], Lr[0], cfg->periodic[0]);
-                if (!mx) continue;
-                unsigned my = FN(axis_images)(t0[1], t1[1], lo[1][q], hi[1][q], Lr[1], cfg->periodic[1]);
-                if (!my) continue;
-                unsigned mz = FN(axis_images)(t0[2], t1[2], lo[2][q], hi[2][q], Lr[2], cfg->periodic[2]);
-                cnt += FN(pop3)[mx] * FN(pop3)[my] * FN(pop3)[mz];
-            }
-            off[r + 1] = cnt;
-            if (cnt > maxrow) maxrow = cnt;
-        }
-        for (int32_t r = 0; r < R; ++r) off[r + 1] += off[r];
-        const int64_t total = off[R];
+        for (int32_t r = 0; r < R; ++r)
+            if (off[r + 1] - off[r] > maxrow) maxrow = off[r + 1] - off[r];
         if (csr_entries) *csr_entries = total;
         int32_t* ent = (int32_t*)malloc((size_t)total * 4 + 4);
         uint8_t* img = (uint8_t*)malloc((size_t)total + 1);
-        if (!ent || !img) { rc = -2; break; }
-
-        /* region search, Write pass: sources in index order, images in (z, y, x) order */
-#pragma omp parallel for schedule(dynamic, 64)
-        for (int32_t r = 0; r < R; ++r) {
-            REAL t0[3], t1[3];
-            for (int a = 0; a < 3; ++a) {
-                t0[a] = lo[a][r] - Hs;
-                t1[a] = hi[a][r] + Hs;
-            }
-            int64_t w = off[r];
-            for (int32_t q = 0; q < R; ++q) {
-                unsigned mx = FN(axis_images)(t0[0], t1[0], lo[0][q], hi[0][q], Lr[0], cfg->periodic[0]);
-                if (!mx) continue;
-                unsigned my = FN(axis_images)(t0[1], t1[1], lo[1][q], hi[1][q], Lr[1], cfg->periodic[1]);
-                if (!my) continue;
-                unsigned mz = FN(axis_images)(t0[2], t1[2], lo[2][q], hi[2][q], Lr[2], cfg->periodic[2]);
-                for (int wz = 0; wz < 3; ++wz)
-                    for (int wy = 0; wy < 3; ++wy)
-                        for (int wx = 0; wx < 3; ++wx)
-                            if (((mx >> wx) & 1u) && ((my >> wy) & 1u) && ((mz >> wz) & 1u)) {
-                                ent[w] = q;
-                                img[w] = (uint8_t)(wx + 3 * wy + 9 * wz);
-                                ++w;
-                            }
-            }
+        if (!ent || !img) {
+            free(ent);
+            free(img);
+            rc = -2;
+            break;
         }
+        FN(region_search)(R, bb, R, bb, Hs, Lr, cfg->periodic, off, ent, img);
 
         /* density pass, one target frame per iteration */
 #pragma omp parallel

@@ -76,6 +76,10 @@ def lib():
             getattr(L, f"orc_brute_counts_{s}").argtypes = [cfgp, C.c_int32, stp]
             getattr(L, f"orc_regions_step_{s}").argtypes = [cfgp, C.c_int32, stp, C.c_int32, C.POINTER(C.c_int64),
                                                             C.c_int32, C.POINTER(C.c_int64), C.POINTER(Conserved)]
+            f = getattr(L, f"orc_region_search_{s}")
+            f.argtypes = [C.c_int32, _vp, C.c_int32, _vp, C.c_double, C.c_double * 3, C.c_int32 * 3,
+                          C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
+            f.restype = C.c_int64
             for nm in ("w", "dwdr"):
                 f = getattr(L, f"orc_kernel_{nm}_{s}")
                 f.argtypes = [C.c_int32, C.c_double, C.c_double]
@@ -247,6 +251,24 @@ class OracleRun:
     def by_id(self):
         """Permutation that orders the current arrays by particle id."""
         return np.argsort(self.id, kind="stable")
+
+
+def region_search(tlo, thi, slo, shi, expand: float, L=(0.0, 0.0, 0.0), periodic=(0, 0, 0), real: str = "f32"):
+    """The oracle's brute-force region search on given boxes ((3, n) lo / hi arrays): CSR offsets,
+    source ids and image codes (wx + 3 wy + 9 wz, w + 1 per axis)."""
+    dt = np.float32 if real == "f32" else np.float64
+    tb = np.ascontiguousarray(np.concatenate([np.asarray(tlo, dt), np.asarray(thi, dt)]))
+    sb = np.ascontiguousarray(np.concatenate([np.asarray(slo, dt), np.asarray(shi, dt)]))
+    nt, ns = tb.shape[1], sb.shape[1]
+    off = np.zeros(nt + 1, dtype=np.int64)
+    fn = getattr(lib(), f"orc_region_search_{real}")
+    args = (nt, tb.ctypes.data, ns, sb.ctypes.data, float(expand), (C.c_double * 3)(*L), (C.c_int32 * 3)(*periodic),
+            off.ctypes.data_as(C.POINTER(C.c_int64)))
+    total = fn(*args, None, None)
+    ent = np.zeros(total, dtype=np.int32)
+    img = np.zeros(total, dtype=np.uint8)
+    assert fn(*args, ent.ctypes.data_as(C.POINTER(C.c_int32)), img.ctypes.data_as(C.POINTER(C.c_uint8))) == total
+    return off, ent, img
 
 
 def regions_layout(cfg: RunConfig, prec: str, ps: ParticleSet, block: int = 1):

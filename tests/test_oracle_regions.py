@@ -7,8 +7,9 @@ force (NeighbourRegions.hpp:55-135) and sweeps every slot of every neighbour fra
 stencil, so it must find the same neighbours. The checks:
   * the layout: one slot per particle, frames aligned to 64, one supercell block per region, id
     order inside a region;
-  * the CSR against a numpy brute-force intersection of the refitted AABBs (the inclusive test of
-    AABB.hpp:116-119 under the periodic images);
+  * the search against the reference's own known answers (FindNeighbourRegions.cpp:41-186), the
+    inclusive bound and the periodic images, and the CSR of a whole step against a numpy brute-force
+    intersection of the refitted AABBs (the inclusive test of AABB.hpp:116-119);
   * neighbour counts bit-exact against the binned stencil and the O(N^2) brute force, with dead
     particles, walls outside a non-periodic box and both precisions with absolute coordinates;
     densities and derivatives within the FP32 summation-order bound of their scales;
@@ -91,6 +92,48 @@ def _numpy_csr_entries(cfg, run: O.RegionRun) -> int:
             cnt *= n_img
         total += int(cnt.sum())
     return total
+
+
+def _cube(lo, hi):
+    """Boxes with the same interval on every axis, as the reference's test sets them."""
+    return np.tile(np.asarray(lo, np.float64), (3, 1)), np.tile(np.asarray(hi, np.float64), (3, 1))
+
+
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_search_known_answers_of_the_reference(oracle_lib, real):
+    """FindNeighbourRegions.cpp:41-186: three regions [0,1], [1.5,2.5], [4,5] searched with
+    H = 0.6 against themselves (offsets 0,2,4,5; ids 0,1 | 0,1 | 2), against a second source
+    [0.2,0.5], [2.8,3], [8,9] (offsets 0,1,2,2; ids 0 | 1), and after moving them to [0,1], [10,11],
+    [20,21] (offsets 0,1,2,3; ids 0 | 1 | 2)."""
+    tlo, thi = _cube([0.0, 1.5, 4.0], [1.0, 2.5, 5.0])
+    off, ent, img = O.region_search(tlo, thi, tlo, thi, 0.6, real=real)
+    assert list(off) == [0, 2, 4, 5] and list(ent) == [0, 1, 0, 1, 2]
+    assert np.all(img == 13)  # no periodic image
+    slo, shi = _cube([0.2, 2.8, 8.0], [0.5, 3.0, 9.0])
+    off, ent, _ = O.region_search(tlo, thi, slo, shi, 0.6, real=real)
+    assert list(off) == [0, 1, 2, 2] and list(ent) == [0, 1]
+    dlo, dhi = _cube([0.0, 10.0, 20.0], [1.0, 11.0, 21.0])
+    off, ent, _ = O.region_search(dlo, dhi, dlo, dhi, 0.6, real=real)
+    assert list(off) == [0, 1, 2, 3] and list(ent) == [0, 1, 2]
+
+
+def test_search_inclusive_bounds_and_periodic_images(oracle_lib):
+    # touching after the expansion counts (AABB.hpp:116-119 is an inclusive test): [0,1] + 0.5 meets [1.5,2]
+    off, ent, _ = O.region_search(*_cube([0.0], [1.0]), *_cube([1.5], [2.0]), 0.5)
+    assert list(off) == [0, 1] and list(ent) == [0]
+    off, _, _ = O.region_search(*_cube([0.0], [1.0]), *_cube([1.5], [2.0]), 0.25)
+    assert list(off) == [0, 0]
+    # periodic x (L = 10): [0.1,0.5] expanded by 0.6 meets [9.6,9.9] only under the image -L
+    tlo, thi = np.array([[0.1], [0.0], [0.0]]), np.array([[0.5], [1.0], [1.0]])
+    slo, shi = np.array([[9.6], [0.0], [0.0]]), np.array([[9.9], [1.0], [1.0]])
+    off, ent, img = O.region_search(tlo, thi, slo, shi, 0.6, L=(10.0, 10.0, 10.0), periodic=(1, 0, 0))
+    assert list(off) == [0, 1] and list(ent) == [0] and list(img) == [0 + 3 * 1 + 9 * 1]
+    off, _, _ = O.region_search(tlo, thi, slo, shi, 0.6, L=(10.0, 10.0, 10.0), periodic=(0, 0, 0))
+    assert list(off) == [0, 0]
+    # a box spanning the whole period meets its neighbour under all three images
+    off, ent, img = O.region_search(*_cube([0.0], [10.0]), *_cube([0.0], [10.0]), 0.5, L=(10.0, 10.0, 10.0),
+                                    periodic=(1, 0, 0))
+    assert list(off) == [0, 3] and sorted(img % 3) == [0, 1, 2]
 
 
 @pytest.mark.parametrize("case", ["kh", "sod_walls"])
