@@ -59,7 +59,10 @@ extern "C" {
 #define SPH_SPECIES_BOUNDARY 1  /* Frozen, Source, Thermodynamic: walls, a source only */
 #define SPH_SPECIES_TRACER 2    /* Movable only: drifted with its own velocity, never a source */
 
-/* Per-run physics and geometry. One run = one independent simulation box. */
+/* Per-run physics and geometry. One run = one independent simulation box. A configuration is
+ * rejected (SPH_E_INVALID, or -1 from the host-only size queries) unless box_lo is finite, box_len,
+ * h0, cfl and cfl_force are finite and > 0, gamma is finite and > 1, dt_fixed and cell_factor are
+ * finite, and the grid has at most 2^30 supercells. */
 typedef struct sph_run_config {
     double box_lo[3];     /* lower corner of the box */
     double box_len[3];    /* edge lengths; periodic axes wrap with this period */

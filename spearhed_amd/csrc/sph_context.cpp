@@ -91,9 +91,15 @@ float sigma_of(int kernel)
 bool make_run(const sph_run_config& c, RunDev& R, std::string& why)
 {
     memset(&R, 0, sizeof(R));
-    if (!(c.h0 > 0.0f)) { why = "h0 must be > 0"; return false; }
+    if (!(c.h0 > 0.0f) || !std::isfinite(c.h0)) { why = "h0 must be finite and > 0"; return false; }
     if (c.kernel < 0 || c.kernel > 2) { why = "unknown kernel id"; return false; }
-    if (!(c.gamma > 1.0f)) { why = "gamma must be > 1"; return false; }
+    if (!(c.gamma > 1.0f) || !std::isfinite(c.gamma)) { why = "gamma must be finite and > 1"; return false; }
+    if (!(c.cfl > 0.0f) || !std::isfinite(c.cfl) || !(c.cfl_force > 0.0f) || !std::isfinite(c.cfl_force)) {
+        why = "cfl and cfl_force must be finite and > 0";
+        return false;
+    }
+    if (std::isnan(c.dt_fixed) || std::isinf(c.dt_fixed)) { why = "dt_fixed must be finite"; return false; }
+    if (std::isnan(c.cell_factor) || std::isinf(c.cell_factor)) { why = "cell_factor must be finite"; return false; }
     R.support = support_of(c.kernel);
     R.H = (float)R.support * c.h0;
     R.H2 = R.H * R.H;
@@ -101,12 +107,17 @@ bool make_run(const sph_run_config& c, RunDev& R, std::string& why)
     int64_t nc = 1;
     for (int a = 0; a < 3; ++a) {
         const double L = c.box_len[a];
-        if (!(L > 0)) { why = "box_len must be > 0"; return false; }
+        if (!(L > 0) || !std::isfinite(L) || !std::isfinite(c.box_lo[a])) {
+            why = "box_len must be finite and > 0, box_lo finite";
+            return false;
+        }
         if (c.periodic[a] && L < (double)R.H * (1.0 + 1e-4)) {
             why = "periodic box edge shorter than the interaction radius";
             return false;
         }
-        int n = (int)std::floor(L / ((double)R.H * f * (1.0 + 1e-4)));
+        const double nd = std::floor(L / ((double)R.H * f * (1.0 + 1e-4)));
+        if (nd > (double)(1 << 30)) { why = "too many supercells"; return false; }
+        int n = (int)nd;
         if (n < 1) n = 1;
         R.n[a] = n;
         R.periodic[a] = c.periodic[a] ? 1 : 0;
@@ -682,6 +693,7 @@ int64_t sph_bytes_per_run(const sph_run_config* cfg, int64_t n, int32_t precisio
     if (!cfg || n < 0) return -1;
     RunDev R;
     std::string why;
+    if (!make_run(*cfg, R, why)) return -1;  // the configuration as given must be valid
     sph_run_config fine = *cfg;
     fine.cell_factor = 1.0f;
     if (!make_run(fine, R, why)) return -1;

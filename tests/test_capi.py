@@ -70,6 +70,25 @@ def test_struct_layouts_match_the_header(tmp_path):
             assert int(out[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)
 
 
+@pytest.mark.parametrize("field,value", [
+    ("h0", float("inf")), ("h0", float("nan")), ("h0", -0.01), ("gamma", float("nan")), ("gamma", 1.0),
+    ("cfl", 0.0), ("cfl", float("nan")), ("cfl_force", -1.0), ("cfl_force", float("inf")),
+    ("dt_fixed", float("inf")), ("dt_fixed", float("nan")), ("cell_factor", float("nan")),
+    ("box_len", (float("inf"), 1.0, 1.0)), ("box_len", (1.0, 0.0, 1.0)), ("box_lo", (0.0, float("nan"), 0.0)),
+    ("box_len", (1e12, 1.0, 1.0)),  # 10^13 supercells on one axis: rejected before any integer conversion
+])
+def test_invalid_run_configurations_are_rejected_on_the_host(field, value):
+    lib = capi.load()
+    cfg = capi.make_config(h0=0.01)
+    assert lib.sph_run_cells(C.byref(cfg)) > 0
+    if isinstance(value, tuple):
+        getattr(cfg, field)[:] = value
+    else:
+        setattr(cfg, field, value)
+    assert lib.sph_run_cells(C.byref(cfg)) == -1
+    assert lib.sph_bytes_per_run(C.byref(cfg), 1000, capi.PREC_MIXED) == -1
+
+
 def test_bytes_per_run_is_host_only():
     lib = capi.load()
     cfg = capi.make_config(h0=0.01)
