@@ -503,6 +503,8 @@ int sph_ctx_create(const sph_run_config* runs, const int64_t* capacity, int32_t 
         const int64_t nc = cells_needed(cfg[r], why);
         if (nc < 0) return fail(c, SPH_E_INVALID, "run " + std::to_string(r) + ": " + why);
         if (sizes[r] < 1) return fail(c, SPH_E_INVALID, "run capacity must be >= 1");
+        if (sizes[r] > (1LL << 30) || cap_p + sizes[r] > (1LL << 30))
+            return fail(c, SPH_E_CAPACITY, "more than 2^30 particles in one context");
         cap_p += sizes[r];
         cap_c += nc;
         cap_kb += (int)((sizes[r] + 255) / 256);
@@ -757,7 +759,8 @@ int sph_upload(sph_ctx* c, int32_t run, const sph_host_soa* s)
         if (dup) return fail(c, SPH_E_INVALID, "upload: particle ids must be unique within a run");
     }
     for (int64_t i = 0; i < s->n; ++i)
-        if (!(s->smoothing_length[i] > 0.0f)) return fail(c, SPH_E_INVALID, "upload: smoothing_length must be > 0");
+        if (!(s->smoothing_length[i] > 0.0f) || !std::isfinite(s->smoothing_length[i]))
+            return fail(c, SPH_E_INVALID, "upload: smoothing_length must be finite and > 0");
     HIPTRY(c, hipStreamSynchronize(c->stream));
     const int64_t n = s->n, o = R.p0;
     StateBuf& S = c->S[c->cur];
@@ -1068,6 +1071,7 @@ int sph_graph_launch(sph_ctx* c, int32_t n_steps)
 {
     if (!c || n_steps < 0) return SPH_E_INVALID;
     if (!c->gexec) return fail(c, SPH_E_STATE, "no captured graph");
+    if (n_steps == 0) return SPH_OK;  // nothing ran: the stage state stays as it is
     if (c->binned) {
         HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_tot, c->stream));
         c->binned = false;

@@ -183,6 +183,23 @@ def test_graph_replay_matches_eager():
     assert ca == cb
 
 
+def test_graph_launch_of_zero_steps_changes_nothing():
+    """A zero-step graph launch after an upload leaves the stage state as the upload left it: the
+    force stage still needs a density pass first."""
+    cfg, ps = _turb(10, 24)
+    with Context([cfg], [ps.n], precision="mixed") as ctx:
+        ctx.upload(0, ps)
+        ctx.prime()
+        ctx.graph_capture()
+        ctx.upload(0, ps)
+        ctx.graph_launch(0)
+        with pytest.raises(capi.SphError, match="density pass"):
+            ctx.forces()
+        ctx.graph_launch(1)  # the eager odd step, then the state is a stepped one again
+        ctx.forces()
+        ctx.sync()
+
+
 def test_batched_runs_equal_alone():
     """Each run of a ragged batch gives bit-identical results to the same run alone."""
     runs = [ic.kelvin_helmholtz(nt, 100 + r) for r, nt in enumerate([4200, 9000, 5100])]
