@@ -101,6 +101,8 @@ int orc_sort(int64_t n, const uint32_t* key, const uint64_t* id, int32_t ncell, 
     int orc_regions_step_##S(const sph_run_config* cfg, int32_t prec, orc_state* st, int32_t n_regions, \
                              const int64_t* region_start, int32_t n_steps, int64_t* csr_entries,   \
                              sph_conserved* out);                                                   \
+    int orc_regions_refit_##S(const sph_run_config* cfg, int32_t prec, const orc_state* st,         \
+                              int32_t n_regions, const int64_t* region_start, void* box);         \
     int64_t orc_region_search_##S(int32_t nt, const void* tbox, int32_t ns, const void* sbox,      \
                                   double expand, const double L[3], const int32_t periodic[3],     \
                                   int64_t* off, int32_t* ent, uint8_t* img);                       \

@@ -916,6 +916,48 @@ int64_t FN(orc_region_search)(int32_t nt, const void* tbox, int32_t ns, const vo
                              img);
 }
 
+/* AABB refit (UpdateRegionBounds, RegionBoundsUpdate.hpp:39-165): each region's box is the min / max
+ * of its live sources' pair coordinates; a region without any gets the empty box (+inf, -inf), which
+ * meets nothing in the search. Boxes in the search's SoA layout. */
+static void FN(region_refit)(const orc_state* st, REAL* const pc[3], int32_t R, const int64_t* region_start, REAL* bb)
+{
+#pragma omp parallel for schedule(static)
+    for (int32_t r = 0; r < R; ++r) {
+        REAL l[3], h[3];
+        for (int a = 0; a < 3; ++a) {
+            l[a] = (REAL)HUGE_VAL;
+            h[a] = (REAL)-HUGE_VAL;
+        }
+        for (int64_t i = region_start[r]; i < region_start[r + 1]; ++i) {
+            if (!is_source(st->species[i], st->mask[i])) continue;
+            for (int a = 0; a < 3; ++a) {
+                if (pc[a][i] < l[a]) l[a] = pc[a][i];
+                if (pc[a][i] > h[a]) h[a] = pc[a][i];
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            bb[(size_t)a * R + r] = l[a];
+            bb[(size_t)(3 + a) * R + r] = h[a];
+        }
+    }
+}
+
+/* The refit on a state laid out in regions (absolute coordinates: plain FP32 or FP64). */
+int FN(orc_regions_refit)(const sph_run_config* cfg, int32_t prec, const orc_state* st, int32_t n_regions,
+                          const int64_t* region_start, void* box)
+{
+    if (prec == ORC_PREC_MIXED || n_regions < 0) return -1;
+    orc_grid g;
+    if (orc_make_grid(cfg, &g)) return -1;
+    REAL* pc[3];
+    for (int a = 0; a < 3; ++a) pc[a] = (REAL*)malloc((size_t)st->n * sizeof(REAL) + 8);
+    if (!pc[0] || !pc[1] || !pc[2]) return -2;
+    FN(pair_coords)(&g, prec, st, pc);
+    FN(region_refit)(st, pc, n_regions, region_start, (REAL*)box);
+    for (int a = 0; a < 3; ++a) free(pc[a]);
+    return 0;
+}
+
 /* The CSR row of region r as spans: each neighbour region's slots under its image shift. */
 static int FN(region_spans)(const int64_t* region_start, const int64_t* off, const int32_t* ent, const uint8_t* img,
                             int32_t r, const REAL Lr[3], FN(span)* sp)
@@ -961,11 +1003,6 @@ int FN(orc_regions_step)(const sph_run_config* cfg, int32_t prec, orc_state* st,
     if (!frame_region || !off || !bb || !pc[0] || !pc[1] || !pc[2] || !pr) return -2;
     for (int32_t r = 0; r < R; ++r)
         for (int64_t f = region_start[r] / 64; f < region_start[r + 1] / 64; ++f) frame_region[f] = r;
-    REAL *lo[3], *hi[3];
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = bb + (size_t)a * R;
-        hi[a] = bb + (size_t)(3 + a) * R;
-    }
 
     int rc = 0;
     sph_conserved c;
@@ -979,26 +1016,7 @@ int FN(orc_regions_step)(const sph_run_config* cfg, int32_t prec, orc_state* st,
         if (rc) break;
         FN(pair_coords)(&g, prec, st, pc);
 
-        /* AABB refit over live sources; a region without any keeps an empty box */
-#pragma omp parallel for schedule(static)
-        for (int32_t r = 0; r < R; ++r) {
-            REAL l[3], h[3];
-            for (int a = 0; a < 3; ++a) {
-                l[a] = (REAL)HUGE_VAL;
-                h[a] = (REAL)-HUGE_VAL;
-            }
-            for (int64_t i = region_start[r]; i < region_start[r + 1]; ++i) {
-                if (!is_source(st->species[i], st->mask[i])) continue;
-                for (int a = 0; a < 3; ++a) {
-                    if (pc[a][i] < l[a]) l[a] = pc[a][i];
-                    if (pc[a][i] > h[a]) h[a] = pc[a][i];
-                }
-            }
-            for (int a = 0; a < 3; ++a) {
-                lo[a][r] = l[a];
-                hi[a][r] = h[a];
-            }
-        }
+        FN(region_refit)(st, pc, R, region_start, bb);
 
         /* region search: Count, scan, Write (every region is both a target and a source) */
         const int64_t total = FN(region_search)(R, bb, R, bb, Hs, Lr, cfg->periodic, off, NULL, NULL);

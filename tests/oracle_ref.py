@@ -76,6 +76,8 @@ def lib():
             getattr(L, f"orc_brute_counts_{s}").argtypes = [cfgp, C.c_int32, stp]
             getattr(L, f"orc_regions_step_{s}").argtypes = [cfgp, C.c_int32, stp, C.c_int32, C.POINTER(C.c_int64),
                                                             C.c_int32, C.POINTER(C.c_int64), C.POINTER(Conserved)]
+            getattr(L, f"orc_regions_refit_{s}").argtypes = [cfgp, C.c_int32, stp, C.c_int32, C.POINTER(C.c_int64),
+                                                             _vp]
             f = getattr(L, f"orc_region_search_{s}")
             f.argtypes = [C.c_int32, _vp, C.c_int32, _vp, C.c_double, C.c_double * 3, C.c_int32 * 3,
                           C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
@@ -319,6 +321,15 @@ class RegionRun(OracleRun):
         assert rc == 0, f"orc_regions_step failed: {rc}"
         self.csr_entries = ce.value
         return out.as_dict()
+
+    def refit(self):
+        """The regions' AABBs refitted to the current positions: (lo, hi), each (3, regions)."""
+        box = np.zeros((6, self.n_regions), dtype=self.real)
+        fn = getattr(lib(), f"orc_regions_refit_{self.sfx}")
+        rc = fn(C.byref(self.cfg), self.pcode, C.byref(self.st), self.n_regions,
+                self.region_start.ctypes.data_as(C.POINTER(C.c_int64)), box.ctypes.data)
+        assert rc == 0, f"orc_regions_refit failed: {rc}"
+        return box[:3], box[3:]
 
     def prime(self) -> dict:
         return self._regions(0)

@@ -117,6 +117,40 @@ def test_search_known_answers_of_the_reference(oracle_lib, real):
     assert list(off) == [0, 1, 2, 3] and list(ent) == [0, 1, 2]
 
 
+def _one_region_run(n=100, seed=0):
+    """n particles in a unit box whose grid is a single supercell (H = 1): one region."""
+    from spearhed_amd import capi
+    from spearhed_amd.particles import ParticleSet
+    cfg = capi.make_config(box_lo=(0, 0, 0), box_len=(1, 1, 1), periodic=(0, 0, 0), kernel=capi.KERNEL_CUBIC,
+                           h0=0.5)
+    ps = ParticleSet(n)
+    ps.id[:] = np.random.default_rng(seed).permutation(n).astype(np.uint64)
+    ps.mass[:] = 1.0
+    ps.h[:] = 0.5
+    ps.u[:] = 1.0
+    return cfg, ps
+
+
+@pytest.mark.parametrize("real", ["plain", "f64"])
+def test_refit_known_answer_of_the_reference(oracle_lib, real):
+    """RegionBoundsUpdate.cpp:43-103: particle 0 at (0.125, 0.125, 0.125), particle 1 at
+    (0.875, 0.875, 0.875), every other at the centre: the refitted box is exactly [0.125, 0.875]^3.
+    A region whose particles are all dead or tracers (no source) gets the empty box."""
+    cfg, ps = _one_region_run()
+    ps.pos[:] = 0.5
+    ps.pos[:, ps.id == 0] = 0.125
+    ps.pos[:, ps.id == 1] = 0.875
+    run = O.RegionRun(cfg, ps, real)
+    assert run.n_regions == 1 and run.n == 128
+    lo, hi = run.refit()
+    assert np.all(lo == 0.125) and np.all(hi == 0.875)
+    ps.mask[:] = 0
+    ps.mask[ps.id == 5] = 1
+    ps.species[ps.id == 5] = 2  # a live tracer is not a source
+    lo, hi = O.RegionRun(cfg, ps, real).refit()
+    assert np.all(np.isposinf(lo)) and np.all(np.isneginf(hi))
+
+
 def test_search_inclusive_bounds_and_periodic_images(oracle_lib):
     # touching after the expansion counts (AABB.hpp:116-119 is an inclusive test): [0,1] + 0.5 meets [1.5,2]
     off, ent, _ = O.region_search(*_cube([0.0], [1.0]), *_cube([1.5], [2.0]), 0.5)
