@@ -12,9 +12,10 @@
  *
  * Pinning: every function is checked by tests/test_oracle_*.py against the reference's
  * known-answer tests (DensitySummation.cpp, HydroForces.cpp, Kernels.cpp,
- * ParticleParticleInteraction.cpp), closed forms (kernel normalisation and derivatives, two-body
- * forces, linear-shear induction, uniform-state invariants), conservation identities, and a
- * brute-force O(N^2) neighbour search. No function is left unpinned.
+ * ParticleParticleInteraction.cpp, and for the region traversal FindNeighbourRegions.cpp and
+ * RegionBoundsUpdate.cpp), closed forms (kernel normalisation and derivatives, two-body forces,
+ * linear-shear induction, uniform-state invariants), conservation identities, and a brute-force
+ * O(N^2) neighbour search. No function is left unpinned.
  */
 #ifndef SPH_ORACLE_H
 #define SPH_ORACLE_H
