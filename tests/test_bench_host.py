@@ -1,0 +1,44 @@
+"""bench.py's host-side contract (CPU only): the one-GPU guard, the two CPU baselines and the
+measured-traffic lookup. The GPU legs run on the GPU box (`python bench.py`)."""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+from spearhed_amd import ic
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+@pytest.mark.parametrize("n", [2, 8])
+def test_more_than_one_gpu_is_refused(n):
+    res = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", str(n)], capture_output=True,
+                         text=True, timeout=120)
+    assert res.returncode == 2
+    assert "one GPU" in res.stderr and res.stdout == ""
+
+
+def test_cpu_baselines_report_the_documented_fields(oracle_lib):
+    cfg, ps = ic.mhd_turbulence(n=12, seed=9)
+    ref = bench.cpu_reference_structured(cfg, ps, 0.2, 2)
+    binned = bench.cpu_baseline(cfg, ps, 0.2)
+    for rec in (ref, binned):
+        assert rec["value"] > 0 and rec["unit"] == "particle-steps/s" and rec["kind"] == "port"
+        assert rec["cores"] >= 1 and f"{ps.n:,}" in rec["sample"]
+    assert "region traversal" in ref["sample"] and "64-slot frames" in ref["sample"]
+    assert "27-supercell" in binned["sample"]
+
+
+def test_measured_traffic_only_for_the_profiled_workload():
+    rec = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))["k_forces"]
+    assert bench.measured_traffic("k_forces", rec["particles"], rec["tuning"]) == rec["bytes_per_launch"]
+    other = dict(rec["tuning"], force_chunk=128 if rec["tuning"]["force_chunk"] != 128 else 64)
+    assert bench.measured_traffic("k_forces", rec["particles"], other) is None
+    assert bench.measured_traffic("k_forces", rec["particles"] + 1, rec["tuning"]) is None
+    assert bench.measured_traffic("k_density", rec["particles"], rec["tuning"]) is None
