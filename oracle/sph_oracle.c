@@ -23,6 +23,15 @@ int orc_num_threads(void)
 #endif
 }
 
+int orc_thread_num(void)
+{
+#ifdef _OPENMP
+    return omp_get_thread_num();
+#else
+    return 0;
+#endif
+}
+
 /* Uniform supercell grid: per axis the largest cell count whose edge is >= cell_factor * H with a
  * 1e-4 relative margin (so FP32 rounding of the cell index can never drop a pair within H).
  * This replaces the reference's irregular region AABBs and their brute-force overlap search

@@ -121,6 +121,7 @@ int orc_regions_layout(const sph_run_config* cfg, int32_t prec, int64_t n, doubl
                        int64_t* n_slots);
 
 int orc_num_threads(void);
+int orc_thread_num(void); /* index of the calling OpenMP thread, < orc_num_threads() */
 
 #ifdef __cplusplus
 }

@@ -1000,11 +1000,13 @@ int FN(orc_regions_step)(const sph_run_config* cfg, int32_t prec, orc_state* st,
     REAL* pc[3];
     for (int a = 0; a < 3; ++a) pc[a] = (REAL*)malloc((size_t)n * sizeof(REAL) + 8);
     FN(prim)* pr = (FN(prim)*)malloc((size_t)n * sizeof(FN(prim)) + 8);
-    if (!frame_region || !off || !bb || !pc[0] || !pc[1] || !pc[2] || !pr) return -2;
-    for (int32_t r = 0; r < R; ++r)
-        for (int64_t f = region_start[r] / 64; f < region_start[r + 1] / 64; ++f) frame_region[f] = r;
-
-    int rc = 0;
+    int rc = (frame_region && off && bb && pc[0] && pc[1] && pc[2] && pr) ? 0 : -2;
+    if (!rc)
+        for (int32_t r = 0; r < R; ++r)
+            for (int64_t f = region_start[r] / 64; f < region_start[r + 1] / 64; ++f) frame_region[f] = r;
+    /* span buffers, one per thread, sized to the longest CSR row of each step */
+    FN(span)* spans = NULL;
+    const int nthr = orc_num_threads();
     sph_conserved c;
     memset(&c, 0, sizeof(c));
     const int prime = (n_steps == 0);
@@ -1026,7 +1028,9 @@ int FN(orc_regions_step)(const sph_run_config* cfg, int32_t prec, orc_state* st,
         if (csr_entries) *csr_entries = total;
         int32_t* ent = (int32_t*)malloc((size_t)total * 4 + 4);
         uint8_t* img = (uint8_t*)malloc((size_t)total + 1);
-        if (!ent || !img) {
+        free(spans);
+        spans = (FN(span)*)malloc((size_t)nthr * (size_t)(maxrow + 1) * sizeof(FN(span)));
+        if (!ent || !img || !spans) {
             free(ent);
             free(img);
             rc = -2;
@@ -1035,21 +1039,17 @@ int FN(orc_regions_step)(const sph_run_config* cfg, int32_t prec, orc_state* st,
         FN(region_search)(R, bb, R, bb, Hs, Lr, cfg->periodic, off, ent, img);
 
         /* density pass, one target frame per iteration */
-#pragma omp parallel
-        {
-            FN(span)* sp = (FN(span)*)malloc((size_t)(maxrow + 1) * sizeof(FN(span)));
-#pragma omp for schedule(dynamic, 16)
-            for (int64_t f = 0; f < nframes; ++f) {
-                const int nsp = FN(region_spans)(region_start, off, ent, img, frame_region[f], Lr, sp);
-                for (int64_t i = f * 64; i < f * 64 + 64; ++i) {
-                    if (!is_target(st->species[i], st->mask[i])) {
-                        if (st->nnb) st->nnb[i] = 0;
-                        continue;
-                    }
-                    FN(density_one)(cfg, st, pc, H2, f0, i, sp, nsp);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t f = 0; f < nframes; ++f) {
+            FN(span)* sp = spans + (size_t)orc_thread_num() * (size_t)(maxrow + 1);
+            const int nsp = FN(region_spans)(region_start, off, ent, img, frame_region[f], Lr, sp);
+            for (int64_t i = f * 64; i < f * 64 + 64; ++i) {
+                if (!is_target(st->species[i], st->mask[i])) {
+                    if (st->nnb) st->nnb[i] = 0;
+                    continue;
                 }
+                FN(density_one)(cfg, st, pc, H2, f0, i, sp, nsp);
             }
-            free(sp);
         }
         st->c_h = FN(clean_speed)(cfg, st);
 
@@ -1057,16 +1057,12 @@ int FN(orc_regions_step)(const sph_run_config* cfg, int32_t prec, orc_state* st,
         FN(prims)(cfg, st, pr);
         FN(fctx) F;
         FN(fctx_init)(cfg, st, H2, pc, pr, &F);
-#pragma omp parallel
-        {
-            FN(span)* sp = (FN(span)*)malloc((size_t)(maxrow + 1) * sizeof(FN(span)));
-#pragma omp for schedule(dynamic, 16)
-            for (int64_t f = 0; f < nframes; ++f) {
-                const int nsp = FN(region_spans)(region_start, off, ent, img, frame_region[f], Lr, sp);
-                for (int64_t i = f * 64; i < f * 64 + 64; ++i)
-                    if (is_target(st->species[i], st->mask[i])) FN(force_one)(&F, st, i, sp, nsp);
-            }
-            free(sp);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t f = 0; f < nframes; ++f) {
+            FN(span)* sp = spans + (size_t)orc_thread_num() * (size_t)(maxrow + 1);
+            const int nsp = FN(region_spans)(region_start, off, ent, img, frame_region[f], Lr, sp);
+            for (int64_t i = f * 64; i < f * 64 + 64; ++i)
+                if (is_target(st->species[i], st->mask[i])) FN(force_one)(&F, st, i, sp, nsp);
         }
         free(ent);
         free(img);
@@ -1075,6 +1071,7 @@ int FN(orc_regions_step)(const sph_run_config* cfg, int32_t prec, orc_state* st,
         if (!rc) st->dt = c.dt_next;
     }
     if (out) *out = c;
+    free(spans);
     free(frame_region);
     free(off);
     free(bb);
