@@ -541,9 +541,10 @@ __device__ __forceinline__ void density_finish(const RunDev& R, int r, StateBuf&
 // ------------------------------------------------------------------------------------------------
 // With the Wendland kernel the pass fits 64 VGPRs without spilling: 8 waves per SIMD instead of 7,
 // 5% faster on the 2M-particle turbulence step. The other kernels keep the compiler's choice
-// (a cap spills them).
+// (a cap spills them), as does the balanced pass with 256-candidate chunks, whose LDS allows 6.
 template <int CH, bool MIXED, bool MHD, int KT, bool BAL>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KT == 2 ? 8 : 1)))  // KT 2: Wendland
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu((KT == 2 && !(BAL && CH > 128)) ? 8 : 1)))  // KT 2: Wendland
 void k_density(const RunDev* __restrict__ runs, int nruns, int ncell_tot, int nblk_total, StateBuf S, WorkBuf W,
                int kmax, int balance)
 {
@@ -737,36 +738,50 @@ __device__ __forceinline__ void force_finish(const RunDev& R, WorkBuf& W, const 
                                              const float4& o4, bool mhd, bool clean, float borve, float ch)
 {
     const int K = lm.K, k = lm.k;
-    // combine the K partial sums in lane order (deterministic), field by field in place
-    auto red = [&](float v) {
-        float s = 0.0f;
-        for (int kk = 0; kk < K; ++kk) s += __shfl(v, lm.gfirst + kk < 64 ? lm.gfirst + kk : 63, 64);
-        return s;
-    };
-    ForceAcc& T = A;
-    T.ax = red(A.ax);
-    T.ay = red(A.ay);
-    T.az = red(A.az);
-    T.DV = red(A.DV);
-    T.AVU = red(A.AVU);
-    {
-        float vm = 0.0f;
-        for (int kk = 0; kk < K; ++kk) vm = fmaxf(vm, __shfl(A.vmax, lm.gfirst + kk < 64 ? lm.gfirst + kk : 63, 64));
-        T.vmax = vm;
+    // combine the K partial sums in lane order (deterministic). All fields of one lane step are
+    // fetched together, so a step's shuffles overlap instead of each waiting for the last (5% of
+    // the force pass at 2M particles against a loop per field; the additions are the same).
+    constexpr int NF = MHD ? 17 : 5;
+    float v[NF] = {A.ax, A.ay, A.az, A.DV, A.AVU};
+    if constexpr (MHD) {
+        const float m[12] = {A.Ti, A.Tj, A.TBx, A.TBy, A.TBz, A.VBx, A.VBy, A.VBz, A.DB, A.PSx, A.PSy, A.PSz};
+#pragma unroll
+        for (int f = 0; f < 12; ++f) v[5 + f] = m[f];
     }
-    if (MHD) {
-        T.Ti = red(A.Ti);
-        T.Tj = red(A.Tj);
-        T.TBx = red(A.TBx);
-        T.TBy = red(A.TBy);
-        T.TBz = red(A.TBz);
-        T.VBx = red(A.VBx);
-        T.VBy = red(A.VBy);
-        T.VBz = red(A.VBz);
-        T.DB = red(A.DB);
-        T.PSx = red(A.PSx);
-        T.PSy = red(A.PSy);
-        T.PSz = red(A.PSz);
+    float s[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) s[f] = 0.0f;
+    float vm = 0.0f;
+    for (int kk = 0; kk < K; ++kk) {
+        const int src = lm.gfirst + kk < 64 ? lm.gfirst + kk : 63;
+        float t[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) t[f] = __shfl(v[f], src, 64);
+        const float tm = __shfl(A.vmax, src, 64);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) s[f] += t[f];
+        vm = fmaxf(vm, tm);
+    }
+    ForceAcc& T = A;
+    T.ax = s[0];
+    T.ay = s[1];
+    T.az = s[2];
+    T.DV = s[3];
+    T.AVU = s[4];
+    T.vmax = vm;
+    if constexpr (MHD) {
+        T.Ti = s[5];
+        T.Tj = s[6];
+        T.TBx = s[7];
+        T.TBy = s[8];
+        T.TBz = s[9];
+        T.VBx = s[10];
+        T.VBy = s[11];
+        T.VBz = s[12];
+        T.DB = s[13];
+        T.PSx = s[14];
+        T.PSy = s[15];
+        T.PSz = s[16];
     }
     if (k == 0 && tgt) {
         const float rho = o2.w;
