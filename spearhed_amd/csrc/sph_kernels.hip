@@ -370,11 +370,17 @@ __device__ __forceinline__ uint32_t cull32(const float4* __restrict__ sp, int e0
 {
     uint32_t m = 0;
     if (lim == 32) {
+        // candidates two at a time, both LDS reads issued before either is used: the scheduler
+        // otherwise waits out every read alone to save registers (density pass 1.5% faster)
 #pragma unroll
-        for (int ii = 0; ii < 32; ++ii) {
+        for (int i0 = 0; i0 < 32; i0 += 2) {
+            const float4 q0 = sp[e0 + i0 * K], q1 = sp[e0 + (i0 + 1) * K];
+            __builtin_amdgcn_sched_barrier(0);
             float dx, dy, dz;
-            const float r2 = geom_r2(xi, yi, zi, sp[e0 + ii * K], dx, dy, dz);
-            m = __builtin_amdgcn_alignbit(m, __float_as_uint(r2 - H2), 31);  // (m << 1) | sign
+            const float r2a = geom_r2(xi, yi, zi, q0, dx, dy, dz);
+            m = __builtin_amdgcn_alignbit(m, __float_as_uint(r2a - H2), 31);  // (m << 1) | sign
+            const float r2b = geom_r2(xi, yi, zi, q1, dx, dy, dz);
+            m = __builtin_amdgcn_alignbit(m, __float_as_uint(r2b - H2), 31);
         }
         return __builtin_bitreverse32(m);
     }
