@@ -35,6 +35,23 @@ def test_cpu_baselines_report_the_documented_fields(oracle_lib):
     assert "27-supercell" in binned["sample"]
 
 
+def test_fp64_error_report(oracle_lib):
+    """The FP64 comparison on a stand-in for the GPU's first pass (the mixed FP32 oracle, whose counts
+    the GPU matches bit for bit): errors within the FP32 bounds, mismatching counts counted."""
+    import oracle_ref as O
+    cfg, ps = ic.mhd_turbulence(n=12, seed=9)
+    o = O.OracleRun(cfg, ps, "mixed")
+    o.resort()
+    o.density()
+    o.forces()
+    g = o.to_particles().sorted_by_id()
+    rep = bench.fp64_error(cfg, ps, g)
+    for key in ("rho", "dvdt", "dudt", "dbdt", "divb"):
+        assert 0.0 <= rep[key] < 1e-5, (key, rep[key])
+    assert rep["neighbour_count_mismatches_fp64"] >= 0
+    assert rep["neighbour_counts_equal_fp64"] == (rep["neighbour_count_mismatches_fp64"] == 0)
+
+
 def test_measured_traffic_only_for_the_profiled_workload():
     rec = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))["k_forces"]
     assert bench.measured_traffic("k_forces", rec["particles"], rec["tuning"]) == rec["bytes_per_launch"]
