@@ -252,7 +252,9 @@ int sph_reduce_conserved(sph_ctx* ctx, sph_conserved* per_run);
 int sph_prime(sph_ctx* ctx);
 /* Whole steps: drift, resort, density, forces, integrate (+ reductions), n_steps times. */
 int sph_step(sph_ctx* ctx, int32_t n_steps);
-/* Capture one whole step of every run into a hipGraph and replay it. */
+/* Capture two whole steps of every run into a hipGraph (a step swaps the double-buffered state,
+ * so two return it to where capture began); sph_graph_launch runs n_steps steps by replaying it,
+ * taking one step eagerly where the parity or an odd count needs it. Results equal sph_step's. */
 int sph_graph_capture(sph_ctx* ctx);
 int sph_graph_launch(sph_ctx* ctx, int32_t n_steps);
 /* Per-particle sort results of the last resort (run-local, current order). */
