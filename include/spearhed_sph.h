@@ -279,7 +279,9 @@ int sph_probe_copy(int64_t bytes, int32_t reps, double* ms_per_copy);
 
 /* ---- autotuner ------------------------------------------------------------------------------ */
 /* Time n_steps steps for each candidate tuning on the context's current data (state restored
- * afterwards) and keep the fastest. Writes the chosen tuning to *best if non-NULL. */
+ * afterwards) and keep the fastest. Writes the chosen tuning to *best if non-NULL. The state is
+ * kept in a device copy of the context's memory for the duration of the search (one more
+ * allocation of the context's size, freed before returning; SPH_E_DEVICE if it does not fit). */
 int sph_autotune(sph_ctx* ctx, int32_t n_steps, sph_tuning* best);
 
 #ifdef __cplusplus
