@@ -110,7 +110,8 @@ static inline void FN(kfactors)(int kernel, REAL h, REAL* invh, REAL* wfac, REAL
         *gfac = sig / (h * h * h * h);
 }
 
-/* gradWScalar with the reference guard (!(r > 0) || r >= support h -> 0), in q = r/h form. */
+/* gradWScalar with the reference guard (!(r > 0) || r >= support h -> 0), in q = r * (1/h) form (the
+ * reference divides x = r / h per pair; the results agree within FP32 rounding). */
 static inline REAL FN(kgrad)(int kernel, REAL rr, REAL invR, REAL q, REAL gfac)
 {
     REAL sup = (REAL)ksupport(kernel);
@@ -324,7 +325,9 @@ static void FN(prims)(const sph_run_config* cfg, const orc_state* st, FN(prim)* 
 }
 
 /* Density of target i summed over the source spans (DensitySummation.hpp:50-138 semantics): the
- * self term plus m_j W(r_ij, h_i) over sources within H, with the neighbour count. */
+ * self term plus m_j W(r_ij, h_i) over sources within H, with the neighbour count. sigma/h^3 is
+ * factored out of the sum and the self term added after the neighbours; the reference seeds rho with
+ * the self term and multiplies each pair by sigma/h^3 (same value within FP32 rounding). */
 static void FN(density_one)(const sph_run_config* cfg, orc_state* st, REAL* const pc[3], REAL H2, REAL f0, int64_t i,
                             const FN(span)* sp, int nsp)
 {
