@@ -173,6 +173,7 @@ def run_serving(args):
         srv.close()
     work = sum(q.n * q.steps for q in reqs)
     return {"requests": len(reqs), "rate_per_s": args.serve_rate, "hbm_budget_GB": args.serve_budget_gb,
+            "pool_reserved_GB": srv.pool_bytes / 2 ** 30, "pool_capacity_particles": srv.max_particles,
             "initial_conditions": "host upload" if args.serve_host_ic else "device lattice blocks",
             "response": "id, pos, vel, rho, u, B + FP64 sums, copied out asynchronously through pinned memory",
             "particles_total": int(sum(q.n for q in reqs)), "step_ms_p50": summary["step_ms_p50"],

@@ -137,8 +137,12 @@ typedef struct sph_grid_info {
 /* Launch tunables chosen by the autotuner (sph_autotune). */
 typedef struct sph_tuning {
     int32_t density_chunk;  /* neighbour candidates staged in LDS per chunk by the density pass: 128|256|512 */
-    int32_t force_chunk;    /* neighbour records staged in LDS per chunk by the force pass: 64|128|256 */
-    int32_t waves_per_cu;   /* persistent-grid waves per CU: 4..32 */
+    int32_t force_chunk;    /* neighbour records staged in LDS per chunk by the force pass: 64|128 */
+    int32_t waves_per_cu;   /* pair passes: 0 = one wave per supercell (default); 1..64 = a persistent
+                               grid of waves_per_cu x (compute units) waves, each walking the supercells
+                               at a stride of the grid (the reference's GridStride schedule,
+                               spmacc/particles/algorithms/FrameSchedule.hpp:92-104, sized as its
+                               FixedGrid policy, FrameDispatch.hpp:137-148) */
     int32_t max_lanes_per_target; /* cap on K, the lanes that share one target's candidates: 1..64 */
     int32_t pair_balance;   /* how the accepted pairs of a target are spread over its K lanes:
                                0: each lane its own candidates; 1: compacted in LDS and shared

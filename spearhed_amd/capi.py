@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libspearhed_sph.so")
 
 SPH_OK = 0
+SPH_E_INVALID, SPH_E_NOMEM, SPH_E_DEVICE, SPH_E_STATE, SPH_E_CAPACITY = -1, -2, -3, -4, -5
 ERRORS = {
     -1: "SPH_E_INVALID",
     -2: "SPH_E_NOMEM",

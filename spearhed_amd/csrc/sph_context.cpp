@@ -40,6 +40,7 @@ struct sph_ctx {
     WorkBuf W = {};
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    int num_cu = 256;  // compute units of the context's device (sizes the persistent pair-pass grid)
     sph_tuning tune = {128, 64, 0, 64, 0};  // fastest measured on the 2M-particle MHD turbulence step
     void* arena = nullptr;
     size_t arena_bytes = 0;
@@ -232,6 +233,11 @@ LaunchCtx lctx(sph_ctx* c)
     L.kern = c->nruns > 0 ? c->cfg[0].kernel : -1;
     for (int r = 1; r < c->nruns; ++r)
         if (c->cfg[r].kernel != L.kern) L.kern = -1;
+    L.persistent_waves = 0;
+    if (c->tune.waves_per_cu > 0) {
+        const int64_t w = (int64_t)c->tune.waves_per_cu * c->num_cu;
+        L.persistent_waves = (int)std::max<int64_t>(8, w / 8 * 8);
+    }
     L.W = c->W;
     L.stream = c->stream;
     return L;
@@ -248,6 +254,7 @@ void invalidate_graph(sph_ctx* c)
 
 int do_bin(sph_ctx* c, double dt, int drift)
 {
+    if (c->nruns == 0) return SPH_OK;  // an empty pool: nothing to launch
     if (c->binned) HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_tot, c->stream));
     LaunchCtx L = lctx(c);
     HIPTRY(c, launch_drift_bin(L, dt, drift));
@@ -259,6 +266,7 @@ int do_bin(sph_ctx* c, double dt, int drift)
 
 int do_resort(sph_ctx* c)
 {
+    if (c->nruns == 0) return SPH_OK;  // an empty pool: nothing to launch
     if (!c->binned) {
         int rc = do_bin(c, 0.0, 0);
         if (rc) return rc;
@@ -275,6 +283,7 @@ int do_resort(sph_ctx* c)
 
 int do_density(sph_ctx* c)
 {
+    if (c->nruns == 0) return SPH_OK;  // an empty pool: nothing to launch
     if (!c->sorted) {
         int rc = do_resort(c);
         if (rc) return rc;
@@ -287,6 +296,7 @@ int do_density(sph_ctx* c)
 
 int do_forces(sph_ctx* c)
 {
+    if (c->nruns == 0) return SPH_OK;  // an empty pool: nothing to launch
     if (!c->dens_valid) return fail(c, SPH_E_STATE, "force pass needs a density pass after the last resort");
     LaunchCtx L = lctx(c);
     HIPTRY(c, launch_forces(L));
@@ -295,6 +305,7 @@ int do_forces(sph_ctx* c)
 
 int do_kick(sph_ctx* c, double dt, int kick)
 {
+    if (c->nruns == 0) return SPH_OK;  // an empty pool: nothing to launch
     LaunchCtx L = lctx(c);
     HIPTRY(c, launch_kick(L, dt, kick));
     return SPH_OK;
@@ -468,6 +479,10 @@ int64_t cells_needed(const sph_run_config& cfg, std::string& why)
 
 int open_stream(sph_ctx* c, void* stream)
 {
+    int dev = 0;
+    HIPTRY(c, hipGetDevice(&dev));
+    HIPTRY(c, hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (c->num_cu < 1) c->num_cu = 1;
     if (stream) {
         c->stream = (hipStream_t)stream;
         return SPH_OK;
@@ -571,7 +586,10 @@ int sph_pool_set_runs(sph_ctx* c, const int32_t* keep, int32_t n_keep, const sph
         cells += nc;
         parts += add_sizes[a];
     }
-    if (parts > c->cap_particles || cells > c->ncell_cap || (int)cfg.size() > c->cap_runs)
+    int64_t blocks = 0;  // particle-kernel blocks of the new run set (each run starts a block)
+    for (int64_t sz : sizes) blocks += (sz + 255) / 256;
+    // every capacity is checked before anything moves: a rejected call leaves the pool as it was
+    if (parts > c->cap_particles || cells > c->ncell_cap || (int)cfg.size() > c->cap_runs || blocks > c->cap_kb)
         return fail(c, SPH_E_CAPACITY, "runs do not fit the pool's capacity");
     // per-run scalars of the kept runs (a kept run not yet stepped still primes on its first step)
     std::vector<double> old_dt(c->nruns), old_time(c->nruns);
@@ -677,6 +695,8 @@ int sph_set_tuning(sph_ctx* c, const sph_tuning* t)
     }
     if (t->max_lanes_per_target < 1 || t->max_lanes_per_target > 64)
         return fail(c, SPH_E_INVALID, "max_lanes_per_target must be in [1, 64]");
+    if (t->waves_per_cu != 0 && (t->waves_per_cu < 1 || t->waves_per_cu > 64))
+        return fail(c, SPH_E_INVALID, "waves_per_cu must be 0 (one wave per supercell) or in [1, 64]");
     c->tune = *t;
     c->dens_valid = false;  // the force pass replays masks laid out for the density pass's tuning
     invalidate_graph(c);
@@ -979,6 +999,7 @@ int sph_stage_integrate(sph_ctx* c, double dt)
 static int read_cons(sph_ctx* c, std::vector<double>& h)
 {
     h.resize((size_t)c->nruns * 8);
+    if (c->nruns == 0) return SPH_OK;
     HIPTRY(c, hipMemcpyAsync(h.data(), c->W.cons, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
     HIPTRY(c, hipStreamSynchronize(c->stream));
     return SPH_OK;
@@ -1184,41 +1205,53 @@ int sph_autotune(sph_ctx* c, int32_t n_steps, sph_tuning* best)
     const sph_tuning t0 = c->tune;
 
     // candidate launch shapes: chunked passes (pair_balance 0/1) and whole-cell passes (2), each at
-    // supercell edges of 1 and 1.15 interaction radii
+    // supercell edges of 1 and 1.15 interaction radii, one wave per supercell; then the winner's
+    // persistent-grid variants (waves per compute unit)
     std::vector<sph_tuning> cands;
     for (int dc : {128, 256})
         for (int fc : {64, 128})
             for (int k : {16, 64})
-                for (int bal : {0, 1}) cands.push_back({dc, fc, t0.waves_per_cu, k, bal});
-    for (int k : {16, 64}) cands.push_back({512, 128, t0.waves_per_cu, k, 2});
+                for (int bal : {0, 1}) cands.push_back({dc, fc, 0, k, bal});
+    for (int k : {16, 64}) cands.push_back({512, 128, 0, k, 2});
     const float cfs[2] = {1.0f, 1.15f};
+    const int wpcs[4] = {4, 8, 16, 32};
 
     double best_t = 1e300;
     sph_tuning best_tune = t0;
     float best_cf = -1.0f;
+    auto time_one = [&](float cf, const sph_tuning& cand) -> int {
+        for (int r = 0; r < c->nruns; ++r) c->cfg[r].cell_factor = std::max(cf, cfg0[r].cell_factor);
+        HIPTRY(c, hipMemcpyAsync(c->arena, sc.backup, bytes, hipMemcpyDeviceToDevice, c->stream));
+        c->cur = cur0;
+        int rc = regrid(c);
+        if (rc) return rc;
+        c->tune = cand;
+        if ((rc = do_step(c))) return rc;  // warm-up
+        HIPTRY(c, hipEventRecord(sc.e0, c->stream));
+        for (int s = 0; s < n_steps; ++s)
+            if ((rc = do_step(c))) return rc;
+        HIPTRY(c, hipEventRecord(sc.e1, c->stream));
+        HIPTRY(c, hipEventSynchronize(sc.e1));
+        float ms = 0;
+        HIPTRY(c, hipEventElapsedTime(&ms, sc.e0, sc.e1));
+        if (ms < best_t) {
+            best_t = ms;
+            best_tune = cand;
+            best_cf = cf;
+        }
+        return SPH_OK;
+    };
     auto search = [&]() -> int {
-        for (float cf : cfs) {
-            for (int r = 0; r < c->nruns; ++r) c->cfg[r].cell_factor = std::max(cf, cfg0[r].cell_factor);
-            for (const sph_tuning& cand : cands) {
-                HIPTRY(c, hipMemcpyAsync(c->arena, sc.backup, bytes, hipMemcpyDeviceToDevice, c->stream));
-                c->cur = cur0;
-                int rc = regrid(c);
-                if (rc) return rc;
-                c->tune = cand;
-                if ((rc = do_step(c))) return rc;  // warm-up
-                HIPTRY(c, hipEventRecord(sc.e0, c->stream));
-                for (int s = 0; s < n_steps; ++s)
-                    if ((rc = do_step(c))) return rc;
-                HIPTRY(c, hipEventRecord(sc.e1, c->stream));
-                HIPTRY(c, hipEventSynchronize(sc.e1));
-                float ms = 0;
-                HIPTRY(c, hipEventElapsedTime(&ms, sc.e0, sc.e1));
-                if (ms < best_t) {
-                    best_t = ms;
-                    best_tune = cand;
-                    best_cf = cf;
-                }
-            }
+        int rc = SPH_OK;
+        for (float cf : cfs)
+            for (const sph_tuning& cand : cands)
+                if ((rc = time_one(cf, cand))) return rc;
+        const sph_tuning shape = best_tune;
+        const float shape_cf = best_cf;
+        for (int w : wpcs) {
+            sph_tuning cand = shape;
+            cand.waves_per_cu = w;
+            if ((rc = time_one(shape_cf, cand))) return rc;
         }
         return SPH_OK;
     };

@@ -19,6 +19,8 @@ struct LaunchCtx {
     int kmax;     // max lanes per target in the pair passes
     int balance;  // share a target's accepted pairs among its K lanes
     int kern;     // smoothing kernel of every run (SPH_KERNEL_*), or -1 when runs differ
+    int persistent_waves;  // pair passes: 0 = one wave per supercell; else a persistent grid of this many
+                           // waves (a multiple of 8) walking the supercells at a stride of the grid
     StateBuf S;          // current buffer set
     WorkBuf W;
     hipStream_t stream;

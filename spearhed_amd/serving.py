@@ -1,12 +1,11 @@
 """Continuous-batching server of independent SPH runs on one GPU.
 
 Requests (a run configuration, its initial particles and a number of steps) arrive over time and
-wait in a FIFO queue. The server keeps one device pool (sph_pool_create) sized from an HBM budget;
-before every batched step it retires runs that have finished their steps (their final state is
-downloaded) and admits waiting requests in arrival order while they fit the budget:
-
-  * the HBM cost of a run is sph_bytes_per_run(cfg, n) (state, work arrays, supercell tables);
-  * the pool's particle / supercell / run capacities bound the resident set as well.
+wait in a FIFO queue. The server keeps one device pool (sph_pool_create) whose whole reservation,
+sph_pool_bytes(particles, supercells, runs), fits the HBM budget: the largest particle capacity P
+with supercell capacity ceil(cells_per_particle * P) that does. Before every batched step it retires
+runs that have finished their steps (their final state is downloaded) and admits waiting requests in
+arrival order while the resident runs fit the pool's particle, supercell and run capacities.
 
 All resident runs then advance one step together (one launch per stage for the whole batch). Every
 run keeps its own dt and time, and its summation orders depend only on its own data, so a run's
@@ -19,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import heapq
+import math
 import time
 from dataclasses import dataclass, field
 from typing import List, Optional
@@ -29,6 +29,34 @@ from . import capi
 from .capi import RunConfig
 from .particles import ParticleSet
 from .stepper import Pool
+
+
+def pool_capacity(budget: int, max_runs: int, cells_per_particle: float, precision: str = "mixed",
+                  mhd: bool = True):
+    """(particles, supercells) of the largest pool whose reservation sph_pool_bytes fits `budget`,
+    with supercells = ceil(cells_per_particle * particles) (at least 64). The reservation grows with
+    the particle capacity, so the largest fitting one is found by bisection."""
+    lib = capi.load()
+    prec = capi.PREC_MIXED if precision == "mixed" else capi.PREC_FP32
+
+    def cells(p):
+        return max(64, int(math.ceil(cells_per_particle * p)))
+
+    def fits(p):
+        b = int(lib.sph_pool_bytes(int(p), cells(p), int(max_runs), prec, int(mhd)))
+        return 0 < b <= budget
+
+    lo = 1024
+    if not fits(lo):
+        raise capi.SphError(f"an HBM budget of {budget} bytes cannot hold a pool of {lo} particles")
+    hi = max(lo, budget // 64)  # no particle costs less than 64 bytes
+    while lo < hi:
+        mid = (lo + hi + 1) // 2
+        if fits(mid):
+            lo = mid
+        else:
+            hi = mid - 1
+    return lo, cells(lo)
 
 
 def run_bytes(cfg: RunConfig, n: int, precision: str = "mixed") -> int:
@@ -115,13 +143,14 @@ class ServeStats:
 class Server:
     """Continuous batching over a fixed device pool.
 
-    hbm_budget: device bytes the resident runs may use (sum of sph_bytes_per_run).
-    cells_per_particle: supercells reserved per budgeted particle (a run with ~57 neighbours uses
-        about 1/14 supercells per particle; the default leaves room for sparser boxes).
+    hbm_budget: device bytes the server may use: the pool's whole reservation fits it.
+    cells_per_particle: supercells reserved per particle of capacity (a uniform run with ~57
+        neighbours uses about 1/14 per particle; a slab one supercell thick about 1/20; a Brio-Wu tube
+        with its eightfold density jump about 1/9).
     """
 
     def __init__(self, hbm_budget: int, precision: str = "mixed", max_runs: int = 512, mhd: bool = True,
-                 cells_per_particle: float = 0.25, pool_factory=Pool, response_fields=None):
+                 cells_per_particle: float = 0.12, pool_factory=Pool, response_fields=None):
         """response_fields: None returns each run's full final state (synchronous download); a tuple of
         names (stepper.RESPONSE_FIELDS) returns only those, copied out asynchronously through pinned
         memory while the pool moves on."""
@@ -129,13 +158,9 @@ class Server:
         self.precision = precision
         self.prec_code = capi.PREC_MIXED if precision == "mixed" else capi.PREC_FP32
         self.budget = int(hbm_budget)
-        # largest particle capacity whose run cost fits the budget: a run's bytes grow by at least
-        # the bytes of one particle of the lightest configuration, so budget / that is an upper bound
-        probe = capi.make_config(h0=0.05, mhd=int(mhd))
-        per_p = max(1, self.lib.sph_bytes_per_run(C.byref(probe), 1 << 20, self.prec_code) // (1 << 20))
-        self.max_particles = max(1024, self.budget // per_p)
-        self.max_cells = max(64, int(self.max_particles * cells_per_particle))
         self.max_runs = int(max_runs)
+        self.max_particles, self.max_cells = pool_capacity(self.budget, self.max_runs, cells_per_particle,
+                                                           precision, mhd)
         self.pool = pool_factory(self.max_particles, self.max_cells, self.max_runs, precision, mhd)
         self.pool_bytes = int(self.lib.sph_pool_bytes(self.max_particles, self.max_cells, self.max_runs,
                                                       self.prec_code, int(mhd)))
@@ -161,8 +186,7 @@ class Server:
         return sum(self.run_bytes(r.req) for r in self.resident)
 
     def submit(self, req: Request):
-        if (self.run_bytes(req) > self.budget or req.n > self.max_particles
-                or self.pool.cells_needed(req.cfg) > self.max_cells):
+        if req.n > self.max_particles or self.pool.cells_needed(req.cfg) > self.max_cells:
             raise capi.SphError(f"request {req.rid} does not fit the server's HBM budget")
         heapq.heappush(self.queue, (req.arrival, req.rid, req))
 
@@ -189,18 +213,17 @@ class Server:
                 self.results[r.req.rid] = Result(r.req.rid, self.pool.download(i), cons[i], r.done, r.req.arrival,
                                                  r.admitted, now)
         self.stats.retire_ms.append((time.perf_counter() - t_r) * 1e3)
-        used_b = sum(self.run_bytes(self.resident[i].req) for i in keep)
         used_p = sum(self.resident[i].req.n for i in keep)
         used_c = sum(self.pool.cells_needed(self.resident[i].req.cfg) for i in keep)
         add: List[Request] = []
         while self.queue and self.queue[0][0] <= admit_until and len(keep) + len(add) < self.max_runs:
             req = self.queue[0][2]
-            b, n, c = self.run_bytes(req), req.n, self.pool.cells_needed(req.cfg)
-            if used_b + b > self.budget or used_p + n > self.max_particles or used_c + c > self.max_cells:
+            n, c = req.n, self.pool.cells_needed(req.cfg)
+            if used_p + n > self.max_particles or used_c + c > self.max_cells:
                 break  # FIFO: the head waits until enough retires
             heapq.heappop(self.queue)
             add.append(req)
-            used_b, used_p, used_c = used_b + b, used_p + n, used_c + c
+            used_p, used_c = used_p + n, used_c + c
         if not finished and not add:
             return
         t_m = time.perf_counter()

@@ -1,4 +1,4 @@
-"""Writes the golden fixtures under tests/golden/ (run: python tests/golden/make_golden.py).
+"""Writes the golden fixtures under tests/golden/ (run: python tests/golden/make_golden.py [case ...]).
 
 Two kinds of fixture:
   * kernels_closed_form.npz -- W(r, h) and dW/dr(r, h) of the cubic M4, quintic M5 and Wendland C2
@@ -106,6 +106,9 @@ CASES = {
     "orszag_tang_2k": lambda: ic.orszag_tang_n(2300, 104),
     "cubic_turbulence_10": lambda: ic.mhd_turbulence(n=10, seed=105, kernel=capi.KERNEL_CUBIC),
     "quintic_turbulence_9": lambda: ic.mhd_turbulence(n=9, seed=106, kernel=capi.KERNEL_QUINTIC),
+    # the serving mix's two other problem types (gamma = 2 and a discontinuous B_y; a dense rotating disc)
+    "brio_wu_3k": lambda: ic.brio_wu(3000, 107),
+    "rotor_3k": lambda: ic.rotor(3000, 108),
 }
 
 
@@ -154,12 +157,13 @@ def make_case(name, path):
     return ps.n
 
 
-def main():
-    make_kernels(os.path.join(HERE, "kernels_closed_form.npz"))
-    for name in CASES:
+def main(names=None):
+    if not names:
+        make_kernels(os.path.join(HERE, "kernels_closed_form.npz"))
+    for name in names or CASES:
         n = make_case(name, os.path.join(HERE, f"{name}.npz"))
         print(f"{name}: {n} particles")
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])  # case names (default: every fixture)

@@ -90,3 +90,61 @@ def test_ensemble_256_sampled_runs_against_the_oracle():
         cfg, ps = runs[r]
         o = oracle_stages(cfg, ps, "mixed")
         compare_forces(got[r], o, mhd=True)
+
+
+def test_turbulence_2m_ten_steps_against_the_oracle():
+    """The benchmark workload stepped: 2,097,152 particles in mixed precision (the configuration's
+    stated precision; its 100 steps are the benchmark's, the first 10 are compared here), primed and
+    stepped 10 times with the GPU's own CFL dt, against the mixed-precision oracle stepped alike
+    (after the multi-step known-answer test of tests/spearhed/unit/SimulationTimestep/
+    SimulationTimestep.cpp:272-330, at full size):
+      * every step: simulated time <= 1e-6 relative, FP64 mass <= 1e-12, energy <= 1e-6, momentum
+        <= 1e-6 of sum m |v|;
+      * after 10 steps, by particle id: density and internal energy <= 1e-5 relative, velocity and B
+        <= 1e-5 of their rms, position <= 1e-5 smoothing lengths (FP32 summation-order differences
+        grow slowly: the FP32 oracle stays within 1.1e-6 of the FP64 one over these steps at 64^3);
+      * the neighbour counts of the last step bit-exact, and its density <= 1e-5, against the
+        oracle's density pass on the GPU's own final positions (the developed, clustered state)."""
+    cfg, ps = _turbulence_2m()
+    steps = 10
+    print(f"full-size turbulence n={ps.n}: {steps} steps on the GPU", flush=True)
+    with Context([cfg], [ps.n], precision="mixed") as ctx:
+        ctx.upload(0, ps)
+        ctx.prime()
+        gcons = []
+        for _ in range(steps):
+            ctx.step(1)
+            gcons.append(ctx.conserved()[0])
+        g = ctx.download(0).sorted_by_id()
+    print("oracle steps", flush=True)
+    o = oracle_ref.OracleRun(cfg, ps, "mixed")
+    o.prime()
+    ocons = [o.step(1) for _ in range(steps)]
+    k = o.by_id()
+    pscale = float(np.sum(o.mass.astype(np.float64) * np.linalg.norm(o.vel.astype(np.float64), axis=0)))
+    for s, (gc, oc) in enumerate(zip(gcons, ocons)):
+        assert abs(gc["time"] - oc["time"]) <= 1e-6 * oc["time"], (s, gc["time"], oc["time"])
+        assert abs(gc["mass"] - oc["mass"]) <= 1e-12 * oc["mass"]
+        assert abs(gc["energy"] - oc["energy"]) <= 1e-6 * abs(oc["energy"]), (s, gc["energy"], oc["energy"])
+        assert max(abs(a - b) for a, b in zip(gc["momentum"], oc["momentum"])) <= 1e-6 * pscale
+    err = {}
+    for f in ("rho", "u"):
+        ref = getattr(o, f)[k].astype(np.float64)
+        err[f] = float(np.max(np.abs(getattr(g, f).astype(np.float64) - ref) / np.abs(ref)))
+    for f in ("vel", "B"):
+        ref = getattr(o, f)[:, k].astype(np.float64)
+        err[f] = float(np.max(np.abs(getattr(g, f).astype(np.float64) - ref)) / np.sqrt(np.mean(ref ** 2)))
+    d = g.pos - o.pos[:, k]
+    d -= np.round(d)  # unit periodic box
+    err["pos/h"] = float(np.max(np.abs(d)) / cfg.h0)
+    print(f"after {steps} steps: " + ", ".join(f"{a} {b:.2e}" for a, b in err.items()), flush=True)
+    for f, e in err.items():
+        assert e <= 1e-5, (f, e)
+    # the last step's neighbour counts and density, on the GPU's own positions
+    r = oracle_ref.OracleRun(cfg, g, "mixed")
+    r.resort()
+    r.density()
+    kr = r.by_id()
+    assert np.array_equal(g.nnb, r.nnb[kr]), "neighbour counts of the developed state differ"
+    rel = np.abs(g.rho.astype(np.float64) - r.rho[kr]) / r.rho[kr]
+    assert float(rel.max()) <= 1e-5

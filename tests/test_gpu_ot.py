@@ -1,10 +1,13 @@
 """GPU long-run parity on the MHD path: the Orszag-Tang slab at its full size.
 
 1,048,576 particles (512 x 512 x 4, SPMHD with Borve correction, hyperbolic div-B cleaning, Monaghan
-viscosity) primed and stepped 100 times in mixed precision with the GPU's own CFL dt, against the
-oracle's run of the same 100 steps (tests/golden/ot_full_100.npz, tests/golden/make_ot_golden.py):
-simulated time, FP64 conserved sums, kinetic / magnetic / thermal energy, the div-B error measure and
-16 x 16 maps of density and B.
+viscosity) primed and stepped with the GPU's own CFL dt, against the oracle's run of the same steps
+in the same precision (tests/golden/make_ot_golden.py):
+  * plain FP32 (FP32 absolute positions) for 200 steps: the configuration's stated precision and
+    step count (tests/golden/ot_fp32_200.npz);
+  * mixed precision for 100 steps (tests/golden/ot_full_100.npz).
+Compared: simulated time, FP64 conserved sums, kinetic / magnetic / thermal energy, the div-B error
+measure and 16 x 16 maps of density and B.
 """
 from __future__ import annotations
 
@@ -19,27 +22,28 @@ from spearhed_amd.stepper import Context
 
 pytestmark = pytest.mark.gpu
 
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ot_full_100.npz")
-STEPS = 100
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # Relative bounds, about 30-100x what was measured (time 4e-8, total energy 2.3e-9, kinetic and thermal
 # energy 4e-10, magnetic energy 2.3e-8, div-B error 1.1e-6, maps 3e-8 of their maximum): at this
 # stage of the vortex the FP32 summation-order differences do not grow into trajectory differences.
 REL = {"time": 1e-6, "energy": 1e-7, "e_kin": 1e-7, "e_th": 1e-7, "e_mag": 1e-6, "divb_err": 1e-4, "maps": 1e-6}
 
 
-def test_orszag_tang_full_size_100_steps_against_the_oracle():
+@pytest.mark.parametrize("fixture,precision,steps", [("ot_fp32_200", "fp32", 200), ("ot_full_100", "mixed", 100)])
+def test_orszag_tang_full_size_against_the_oracle(fixture, precision, steps):
     cfg, ps = ic.orszag_tang(n=512, nz=4, seed=2)
-    print(f"Orszag-Tang n={ps.n}", flush=True)
-    with Context([cfg], [ps.n], precision="mixed") as ctx:
+    print(f"Orszag-Tang n={ps.n} {precision} {steps} steps", flush=True)
+    with Context([cfg], [ps.n], precision=precision) as ctx:
         ctx.upload(0, ps)
         ctx.prime()
-        for _ in range(STEPS // 10):
+        for _ in range(steps // 10):
             ctx.step(10)
         ctx.sync()
         c = ctx.conserved()[0]
         g = ctx.download(0)
     st = mhd_stats.stats(g)
-    z = np.load(GOLDEN)
+    z = np.load(os.path.join(GOLDEN, f"{fixture}.npz"))
+    assert int(z["steps"]) == steps
     rel = {k: abs(st[k] - float(z[k])) / abs(float(z[k])) for k in ("e_kin", "e_mag", "e_th", "divb_err")}
     maps = {k: float(np.max(np.abs(st[k] - z[k])) / np.max(np.abs(z[k]))) for k in ("map_rho", "map_bx", "map_by")}
     print(f"OT: t {c['time']:.9f} vs {float(z['time']):.9f}; energy {c['energy']:.12e} vs {float(z['energy']):.12e}; "

@@ -243,6 +243,12 @@ MODES = {
     "cell512": dict(density_chunk=512, force_chunk=128, max_lanes_per_target=64, pair_balance=2),
     "cell512k16": dict(density_chunk=512, force_chunk=128, max_lanes_per_target=16, pair_balance=2),
     "chunked128": dict(density_chunk=256, force_chunk=128, max_lanes_per_target=16, pair_balance=0),
+    # persistent grids of one wave per compute unit: cases with more supercells than waves stride
+    "chunked_grid": dict(density_chunk=128, force_chunk=64, max_lanes_per_target=64, pair_balance=0, waves_per_cu=1),
+    "balanced_grid": dict(density_chunk=256, force_chunk=128, max_lanes_per_target=16, pair_balance=1,
+                          waves_per_cu=1),
+    "cell512_grid": dict(density_chunk=512, force_chunk=128, max_lanes_per_target=64, pair_balance=2,
+                         waves_per_cu=1),
 }
 
 
@@ -338,3 +344,50 @@ def test_hydro_run_batched_with_mhd_runs_equals_alone():
     for a, b in zip(alone, got):
         for f in ("pos", "vel", "u", "rho", "dvdt", "dudt", "vsig", "nnb"):
             assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+@pytest.mark.parametrize("wpc", [1, 4, 16])
+def test_persistent_grid_batch_matches_oracle_and_alone(wpc):
+    """A ragged batch of Kelvin-Helmholtz runs (about 6,800 supercells) stepped with a persistent
+    pair-pass grid of wpc waves per compute unit, each wave striding over supercells: every run
+    meets the oracle contract at the first pass, and after two steps the batch is bit-identical to
+    the same batch with one wave per supercell and to each run alone."""
+    sizes = [4096, 9000, 20000, 6000, 12000, 30000, 5000, 8000]
+    runs = [ic.kelvin_helmholtz(nt, 900 + r) for r, nt in enumerate(sizes)]
+    cfgs, n = [c for c, _ in runs], [p.n for _, p in runs]
+    tune = dict(density_chunk=128, force_chunk=64, max_lanes_per_target=64, pair_balance=0)
+
+    def batch(waves, steps):
+        with Context(cfgs, n, precision="mixed") as ctx:
+            ctx.set_tuning(**tune, waves_per_cu=waves)
+            for r, (_, p) in enumerate(runs):
+                ctx.upload(r, p)
+            if steps:
+                ctx.prime()
+                ctx.step(steps)
+            else:
+                ctx.resort()
+                ctx.density()
+                ctx.forces()
+            ctx.sync()
+            return [ctx.download(r).sorted_by_id() for r in range(len(runs))]
+
+    print(f"persistent grid wpc={wpc} runs={len(runs)} particles={sum(n)}", flush=True)
+    first = batch(wpc, 0)
+    for r in (0, 2, 5):
+        compare_forces(first[r], oracle_stages(*runs[r], "mixed"), mhd=True)
+    stepped = batch(wpc, 2)
+    for a, b in zip(batch(0, 2), stepped):
+        for f in ("pos", "vel", "u", "rho", "B", "psi", "dvdt", "dbdt", "nnb"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    for r in (1, 5):
+        cfg, ps = runs[r]
+        with Context([cfg], [ps.n], precision="mixed") as ctx:
+            ctx.set_tuning(**tune, waves_per_cu=wpc)
+            ctx.upload(0, ps)
+            ctx.prime()
+            ctx.step(2)
+            ctx.sync()
+            alone = ctx.download(0).sorted_by_id()
+        for f in ("pos", "vel", "u", "rho", "B", "dvdt", "nnb"):
+            assert np.array_equal(getattr(alone, f), getattr(stepped[r], f)), (r, f)

@@ -209,3 +209,42 @@ def test_served_response_fields_equal_alone():
             assert np.array_equal(got.fields[f][..., order], getattr(a, f)), f
         for k in ("mass", "energy", "time", "dt_next"):
             assert got.conserved[k] == ca[k], k
+
+
+def test_set_runs_rejects_block_overflow_and_leaves_runs_intact():
+    """A context from sph_ctx_create has exactly the particle-kernel blocks of its runs: two runs of
+    256 particles hold 2 blocks, so a new set of 1 + 511 particles (3 blocks) fits the particle count
+    but not the blocks. The call must fail before anything moves and leave both runs as they were."""
+    import ctypes as C
+
+    from spearhed_amd import capi
+
+    cfg, ps = ic.kelvin_helmholtz(256, 71)
+    ps = ps.take(np.arange(256))
+    ps.id = np.arange(256, dtype=np.uint64)
+    with Context([cfg, cfg], [256, 256], precision="mixed") as ctx:
+        ctx.upload(0, ps)
+        ctx.upload(1, ps)
+        ctx.prime()
+        ctx.step(1)
+        ctx.sync()
+        before = [ctx.download(r).sorted_by_id() for r in range(2)]
+        add = (capi.RunConfig * 2)(cfg, cfg)
+        sizes = (C.c_int64 * 2)(1, 511)
+        rc = ctx.lib.sph_pool_set_runs(ctx.h, None, 0, add, sizes, 2)
+        assert rc == capi.SPH_E_CAPACITY
+        after = [ctx.download(r).sorted_by_id() for r in range(2)]
+        for a, b in zip(before, after):
+            _same(a, b)
+        ctx.step(1)  # still steps
+        ctx.sync()
+
+
+def test_empty_pool_stages_are_no_ops():
+    with Pool(10000, 1000, 4, "mixed", True) as pool:
+        for stage in ("drift", "resort", "density", "forces"):
+            getattr(pool, stage)()
+        pool.integrate()
+        pool.step(2)
+        assert pool.conserved() == []
+        pool.sync()

@@ -314,3 +314,160 @@ def test_fp32_oracle_close_to_fp64(oracle_lib):
     assert np.max(np.abs(a.rho[oa] - b.rho[ob]) / b.rho[ob]) < 1e-5
     assert np.max(np.abs(a.dvdt[:, oa] - b.dvdt[:, ob]) / b.sc_a[ob]) < 1e-5
     assert np.max(np.abs(a.dbdt[:, oa] - b.dbdt[:, ob]) / b.sc_db[ob]) < 1e-5
+
+
+# -- magnitudes of the terms the reference lacks ---------------------------------------------------
+def _wendland_s(r, h):
+    """Wendland C2 gradient scalar s = (dW/dr) / r with support 2h (Dehnen & Aly 2012)."""
+    q = r / h
+    return -(105.0 / (16 * math.pi * h ** 5)) * (1 - q / 2) ** 3
+
+
+def test_viscosity_two_particle_closed_form(oracle_lib):
+    """One approaching pair, equal m, h, rho, c (Monaghan 1997, JCP 136:298, eq. 4.1-4.2):
+    mu = h v_ij.r_ij / (r^2 + 0.01 h^2), Pi = (-alpha c mu + beta mu^2) / rho,
+    dv_i/dt = -m Pi grad_i W, du_i/dt = 1/2 m Pi v_ij.grad_i W. The viscous part is the difference of
+    the run with alpha = 1, beta = 2 and the inviscid run."""
+    h, d, v0, gam, u = 0.5, 0.625, 0.75, 1.5, 1.25  # exact in FP32
+    res = {}
+    for alpha in (0.0, 1.0):
+        cfg = capi.make_config(box_lo=(-2, -2, -2), box_len=(4, 4, 4), periodic=(0, 0, 0),
+                               kernel=capi.KERNEL_WENDLAND, h0=h, gamma=gam, alpha_visc=alpha, beta_visc=2.0)
+        pos = np.zeros((3, 2))
+        pos[0] = [-d / 2, d / 2]
+        ps = _set(pos, 1.0, h, u=u)
+        ps.vel[0] = [v0, -v0]  # approaching
+        o, order = _forces(cfg, ps, "f64", rho=1.0)
+        res[alpha] = (o.dvdt[:, order].copy(), o.dudt[order].copy())
+    da = res[1.0][0] - res[0.0][0]
+    du = res[1.0][1] - res[0.0][1]
+    c = math.sqrt(gam * (gam - 1) * u)  # rho = 1: c^2 = gamma P / rho
+    rx, vr = -d, (2 * v0) * (-d)        # r_ij = x_i - x_j, v_ij . r_ij of the left particle
+    mu = h * vr / (d * d + 0.01 * h * h)
+    Pi = (-1.0 * c * mu + 2.0 * mu * mu) / 1.0
+    s = _wendland_s(d, h)
+    ax_left = -1.0 * Pi * s * rx
+    du_each = 0.5 * 1.0 * Pi * s * vr
+    np.testing.assert_allclose(da[0], [ax_left, -ax_left], rtol=1e-12)
+    np.testing.assert_allclose(da[1:], 0.0, atol=1e-15)
+    np.testing.assert_allclose(du, [du_each, du_each], rtol=1e-12)
+    assert ax_left < 0 < du_each
+
+
+def test_cleaning_grad_psi_of_a_linear_field(oracle_lib):
+    """psi = g x, B = v = 0 on a lattice: the cleaning term of the induction equation is
+    dB/dt = -rho grad psi with the symmetric SPH gradient rho sum m (psi_i/rho_i^2 + psi_j/rho_j^2)
+    grad W, which on a perfect lattice is exactly -g G/rho along x (G/rho: the exact lattice sum
+    of the difference-form derivative, _lattice_gradient_factor)."""
+    n = 16
+    pos, m, h, lo, L = _lattice_box(n, periodic=False)
+    cfg = capi.make_config(box_lo=lo, box_len=L, periodic=(0, 0, 0), kernel=capi.KERNEL_WENDLAND, h0=h, mhd=1,
+                           cleaning=1, alpha_visc=0.0)
+    g = 0.25  # psi = (2i + 1)/128: exact in FP32
+    ps = _set(pos, m, h, rho=1.0, u=1.5)
+    ps.psi[:] = g * pos[0]
+    o, order = _forces(cfg, ps, "f64")
+    inner = _interior(pos, 2.5 * 2 * h)
+    assert inner.sum() > 20
+    fac = _lattice_gradient_factor(h, 1.0 / n)
+    db = o.dbdt[:, order][:, inner]
+    np.testing.assert_allclose(db[0], -g * fac, rtol=1e-9)
+    np.testing.assert_allclose(db[1:], 0.0, atol=1e-12)
+
+
+def test_cleaning_psi_div_v_of_a_uniform_expansion(oracle_lib):
+    """v = a x (uniform expansion), uniform psi, B = 0 on a lattice: div B = 0 and the SPH div v is
+    3 a G/rho, so dpsi/dt = -psi sigma c_h / h - 1/2 psi div v, with c_h = sqrt(gamma (gamma-1) u)."""
+    n = 16
+    pos, m, h, lo, L = _lattice_box(n, periodic=False)
+    gam, sig, u, psi0, a = float(np.float32(5.0 / 3.0)), 0.75, 1.5, 0.125, 0.5
+    cfg = capi.make_config(box_lo=lo, box_len=L, periodic=(0, 0, 0), kernel=capi.KERNEL_WENDLAND, h0=h, gamma=gam,
+                           mhd=1, cleaning=1, clean_sigma=sig, alpha_visc=0.0)
+    ps = _set(pos, m, h, rho=1.0, u=u)
+    ps.vel[:] = a * pos  # a x = (2i + 1)/64: exact in FP32
+    ps.psi[:] = psi0
+    o, order = _forces(cfg, ps, "f64")
+    ch = math.sqrt(gam * (gam - 1) * u)
+    assert o.c_h == pytest.approx(ch, rel=1e-12)
+    inner = _interior(pos, 2.5 * 2 * h)
+    fac = _lattice_gradient_factor(h, 1.0 / n)
+    divv = 3 * a * fac
+    expected = -psi0 * sig * ch / h - 0.5 * psi0 * divv
+    np.testing.assert_allclose(o.dpsidt[order][inner], expected, rtol=1e-9)
+    np.testing.assert_allclose(o.divb[order][inner], 0.0, atol=1e-12)
+
+
+def _divergent_field(pos, rng, amp):
+    """B = grad phi of a random periodic potential (curl-free: all of it is divergence)."""
+    k = rng.integers(1, 3, size=(4, 3)) * 2 * math.pi
+    ph = rng.uniform(0, 2 * math.pi, size=4)
+    B = np.zeros_like(pos)
+    for kk, p in zip(k, ph):
+        s = np.cos(kk @ pos + p)
+        B += -amp * kk[:, None] * np.sin(kk @ pos + p)[None, :] / np.linalg.norm(kk)
+        del s
+    return B
+
+
+def test_cleaning_energy_rate_identity(oracle_lib):
+    """Tricco & Price (2012): with the difference-form div B and the symmetric grad psi the hyperbolic
+    cleaning terms exchange energy exactly, so at rest (v = 0) the magnetic + cleaning energy
+    E = sum (m/rho) (B^2/2 + psi^2/(2 c_h^2)) changes only by the damping term:
+    dE/dt = sum (m/rho) (B.dB/dt + psi dpsi/dt / c_h^2) = -sum (m/rho) psi^2 sigma / (h c_h).
+    Holds on any particle distribution (the pair terms cancel pair by pair)."""
+    n = 10
+    dx = 1.0 / n
+    h = float(np.float32(ic.h_for_spacing(dx)))
+    rng = np.random.default_rng(21)
+    pos = ic.lattice(n, n, n, (0, 0, 0), dx, rng, jitter=0.2)
+    sig = 0.75  # exact in the configuration's FP32 field
+    cfg = capi.make_config(periodic=(1, 1, 1), kernel=capi.KERNEL_WENDLAND, h0=h, mhd=1, cleaning=1, borve=0,
+                           clean_sigma=sig)
+    ps = _set(pos, float(np.float32(dx ** 3)), h, rho=1.0, u=1.0)
+    ps.u[:] = rng.uniform(0.5, 2.0, ps.n)
+    ps.B[:] = _divergent_field(pos, rng, 0.3)
+    ps.psi[:] = rng.normal(0, 0.05, ps.n)
+    o, order = _forces(cfg, ps, "f64")
+    w = o.mass / o.rho
+    ch = o.c_h
+    rate = np.sum(w * (np.sum(o.B * o.dbdt, axis=0) + o.psi * o.dpsidt / ch ** 2))
+    damping = -np.sum(w * o.psi ** 2 * sig / (o.h * ch))
+    scale = np.sum(w * (np.linalg.norm(o.B, axis=0) * np.linalg.norm(o.dbdt, axis=0)
+                        + np.abs(o.psi * o.dpsidt) / ch ** 2))
+    assert abs(rate - damping) <= 1e-12 * scale, (rate, damping, scale)
+    assert abs(damping) > 1e-3 * scale  # the identity is not met trivially
+
+
+def test_cleaning_energy_decays_over_steps(oracle_lib):
+    """The same energy over whole oracle steps on a seeded divergent field (fluid at rest on a
+    perfect lattice, B small enough that the Lorentz force leaves it at rest, dt well inside the
+    Courant limit of the cleaning wave). It falls monotonically, as Tricco & Price (2012) show, once
+    psi has built up from zero: in the first steps the damping, which acts on psi, is still smaller
+    than the O(dt^2) energy gain of the explicit kick, which acts on the whole field."""
+    n = 10
+    dx = 1.0 / n
+    h = float(np.float32(ic.h_for_spacing(dx)))
+    rng = np.random.default_rng(22)
+    pos = ic.lattice(n, n, n, (0, 0, 0), dx, None)
+    gam, u = float(np.float32(5.0 / 3.0)), 1.5
+    ch = math.sqrt(gam * (gam - 1) * u)
+    dt = 0.05 * h / ch
+    cfg = capi.make_config(periodic=(1, 1, 1), kernel=capi.KERNEL_WENDLAND, h0=h, gamma=gam, mhd=1, cleaning=1,
+                           clean_sigma=1.0, alpha_visc=0.0, dt_fixed=dt)
+    ps = _set(pos, float(np.float32(dx ** 3)), h, rho=1.0, u=u)
+    ps.B[:] = _divergent_field(pos, rng, 1e-3)
+    o = O.OracleRun(cfg, ps, "f64")
+    o.prime()
+
+    def energy():
+        return float(np.sum(o.mass / o.rho * (0.5 * np.sum(o.B * o.B, axis=0) + 0.5 * o.psi ** 2 / o.c_h ** 2)))
+
+    e = [energy()]
+    for _ in range(40):
+        o.step(1)
+        e.append(energy())
+    e = np.array(e)
+    assert e[1] <= e[0] * (1 + 1e-2)  # the transient stays small
+    assert np.all(np.diff(e[5:]) < 0), np.diff(e) / e[0]
+    assert e[-1] < 0.5 * e[0], e[-1] / e[0]
+    assert np.max(np.abs(o.vel)) < 1e-6  # the fluid stayed at rest

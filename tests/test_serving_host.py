@@ -5,8 +5,16 @@ from __future__ import annotations
 
 import pytest
 
+import ctypes as C
+
 from spearhed_amd import capi, ic
-from spearhed_amd.serving import Request, Server, poisson_arrivals, run_bytes
+from spearhed_amd.serving import Request, Server, poisson_arrivals, pool_capacity
+
+
+def _budget_for(particles, max_runs=512, cpp=0.12):
+    """The reservation of a pool of `particles` capacity (the server's default cell ratio)."""
+    cells = max(64, int(-(-cpp * particles // 1)))
+    return int(capi.load().sph_pool_bytes(int(particles), cells, max_runs, capi.PREC_MIXED, 1))
 
 
 class RecorderPool:
@@ -59,17 +67,31 @@ def _requests(k, steps, n_target=2000):
     return out
 
 
+@pytest.mark.parametrize("budget_gb", [0.01, 0.5, 4.0, 64.0])
+def test_pool_reservation_fits_the_budget(budget_gb):
+    budget = int(budget_gb * 2 ** 30)
+    srv = Server(budget, max_runs=256, pool_factory=RecorderPool)
+    reserved = capi.load().sph_pool_bytes(srv.max_particles, srv.max_cells, srv.max_runs, capi.PREC_MIXED, 1)
+    assert srv.pool_bytes == reserved
+    assert 0 < reserved <= budget
+    # the largest such pool: one more particle of capacity does not fit
+    p, c = pool_capacity(budget, 256, 0.12)
+    assert (p, c) == (srv.max_particles, srv.max_cells)
+    assert _budget_for(p + 1, 256) > budget
+    assert reserved > 0.99 * budget - 64 * 2 ** 10
+
+
 def test_budget_limits_the_resident_set():
     reqs = _requests(6, [3, 5])
-    one = run_bytes(reqs[0].cfg, reqs[0].ps.n)
-    srv = Server(int(2.5 * one), pool_factory=RecorderPool)
+    srv = Server(_budget_for(int(2.5 * reqs[0].ps.n)), pool_factory=RecorderPool)
     res_sizes = []
     orig = srv._retire_and_admit
 
     def spy(until):
         orig(until)
         res_sizes.append(len(srv.resident))
-        assert srv.resident_bytes() <= srv.budget
+        assert sum(r.req.n for r in srv.resident) <= srv.max_particles
+        assert srv.pool_bytes <= srv.budget
 
     srv._retire_and_admit = spy
     srv.serve(reqs, realtime=False)
@@ -93,7 +115,7 @@ def test_fifo_and_retire_order():
 
 def test_request_larger_than_budget_is_rejected():
     reqs = _requests(1, [1])
-    srv = Server(run_bytes(reqs[0].cfg, reqs[0].ps.n) - 1, pool_factory=RecorderPool)
+    srv = Server(_budget_for(reqs[0].ps.n - 300), pool_factory=RecorderPool)  # 300: beyond 256-byte padding
     with pytest.raises(capi.SphError):
         srv.submit(reqs[0])
 
