@@ -41,6 +41,13 @@ DENSITY_FLOP_PER_PAIR = 15
 # Algorithmic HBM bytes of the force pass per particle: read the 5 float4 interaction records once,
 # write dv/dt (12), du/dt, dB/dt (12), dpsi/dt, div B, v_sig (16).
 FORCE_BYTES_PER_PARTICLE = 80 + 44
+DENSITY_FLOP_PER_PARTICLE = 49
+# Algorithmic bytes per particle of the other stages of a mixed-precision MHD step (each field read and
+# written once): drift + key + histogram rank, scatter, gather into the sorted set (+ pair record),
+# density (positions, masses, h in; rho, counts and the force pass's records out), kick + sums.
+STAGE_BYTES_PER_PARTICLE = {"drift_bin": 70, "resort": 32 + 190, "density": 130, "forces": FORCE_BYTES_PER_PARTICLE,
+                            "integrate": 110}
+CONFIG_STEPS = 100  # the 2M-particle turbulence configuration's step count
 
 
 def measured_traffic(kernel: str, particles: int, tuning: dict):
@@ -56,9 +63,24 @@ def measured_traffic(kernel: str, particles: int, tuning: dict):
     return rec["bytes_per_launch"]
 
 
+def step_roofline(n: int, pairs: int, step_ms: float) -> dict:
+    """The whole step against its speed of light: each stage bounded by the larger of its FP32 work at
+    the FP32 peak and its algorithmic bytes at the HBM peak, summed over the stages."""
+    flop = {"density": DENSITY_FLOP_PER_PAIR * pairs + DENSITY_FLOP_PER_PARTICLE * n,
+            "forces": FORCE_FLOP_PER_PAIR * pairs + FORCE_FLOP_PER_PARTICLE * n}
+    sol = {}
+    for stage, b in STAGE_BYTES_PER_PARTICLE.items():
+        sol[stage] = max(flop.get(stage, 0) / (FP32_PEAK_TFLOPS * 1e12), b * n / (HBM_PEAK_GBS * 1e9)) * 1e3
+    total = sum(sol.values())
+    return {"sol_ms": total, "stage_sol_ms": sol, "measured_ms": step_ms, "frac": total / step_ms,
+            "basis": "sum over stages of max(FP32 FLOP / 157.3 TFLOP/s, algorithmic bytes / 8 TB/s)"}
+
+
 def run_main(args):
     t0 = time.perf_counter()
     cfg, ps = ic.mhd_turbulence(n=args.n, seed=3)
+    if args.cell_factor:
+        cfg.cell_factor = args.cell_factor
     n = ps.n
     t_ic = time.perf_counter() - t0
     ctx = Context([cfg], [n], precision=args.precision)
@@ -83,9 +105,22 @@ def run_main(args):
     pairs = ctx.neighbour_total(0)
     cons = ctx.conserved()[0]
     tuning = ctx.tuning
+    # the configuration's whole run: the same initial state primed and stepped CONFIG_STEPS times (the
+    # pair passes slow down as the turbulent state develops, so the first steps flatter the run)
+    ctx.upload(0, ps)
+    ctx.prime()
+    ctx.sync()
+    ctx.profile_begin(CONFIG_STEPS)
+    t3 = time.perf_counter()
+    ctx.step(CONFIG_STEPS)
+    ctx.sync()
+    t4 = time.perf_counter()
+    full = ctx.profile_end()
+    full["wall_s"] = t4 - t3
+    full["pairs_last"] = ctx.neighbour_total(0)
     ctx.close()
     return dict(n=n, wall=wall, prof=prof, pairs=pairs, cons=cons, t_ic=t_ic, cfg=cfg, ps=ps, tuning=tuning,
-                primed=primed)
+                primed=primed, full=full)
 
 
 def run_ensemble(args):
@@ -184,7 +219,9 @@ def run_serving(args):
             "request_latency_ms_p99": float(np.percentile(lat, 99)),
             "admission_and_upload_ms_total": summary["admission_ms_total"],
             "retire_ms_total": summary["retire_ms_total"], "move_ms_total": summary["move_ms_total"],
-            "fill_ms_total": summary["fill_ms_total"], "wall_s": wall}
+            "fill_ms_total": summary["fill_ms_total"], "harvest_ms_total": summary["harvest_ms_total"],
+            "admission_ms_p50": summary["admission_ms_p50"], "steps": summary["steps"], "wall_s": wall,
+            "device_busy_s": summary["steps"] and sum(srv.stats.step_ms) * 1e-3}
 
 
 def fp64_error(cfg, ps, g):
@@ -280,6 +317,9 @@ def main():
     ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
     ap.add_argument("--tuning", default="", help="JSON launch shape for the main workload, e.g. "
                     "'{\"pair_balance\": 2, \"density_chunk\": 512, \"force_chunk\": 256}'")
+    ap.add_argument("--cell-factor", type=float, default=0.0,
+                    help="supercell edge in interaction radii for the main workload (0: the default, 1; 0.5: the "
+                         "125-supercell stencil)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of each CPU figure")
     ap.add_argument("--cpu-region-block", type=int, default=2,
@@ -287,7 +327,7 @@ def main():
                          "of 1, 2, 3 at 2,097,152 particles: the region search is quadratic in the region count)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-runs", type=int, default=256)
-    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--ensemble-steps", type=int, default=50)
     ap.add_argument("--no-tune-ensemble", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=64)
@@ -345,6 +385,14 @@ def main():
         },
         "conserved": m["cons"],
     }
+    out["step_roofline"] = step_roofline(n, pairs, m["prof"]["step_ms_mean"])
+    full = m["full"]
+    out["configured_run"] = {
+        "steps": CONFIG_STEPS, "ms_per_step_mean": full["step_ms_mean"], "step_ms_p50": full["step_ms_p50"],
+        "step_ms_p99": full["step_ms_p99"], "stage_ms": full["stage_ms"],
+        "particle_steps_per_s": n * CONFIG_STEPS / full["wall_s"],
+        "neighbours_mean_last_step": full["pairs_last"] / n,
+        "note": "the same initial state primed and stepped 100 times: the configuration's whole run"}
     _progress(f"main workload: {value:.4g} particle-steps/s")
     if not args.no_ensemble:
         out["ensemble"] = run_ensemble(args)

@@ -82,7 +82,9 @@ typedef struct sph_run_config {
     float cfl;            /* Courant factor on h / v_sig (reference sph.param cflFactor) */
     float cfl_force;      /* factor on sqrt(h / |a|) */
     double dt_fixed;      /* > 0: constant dt (reference dimension.param dt); <= 0: CFL dt */
-    float cell_factor;    /* supercell edge >= cell_factor * H (>= 1) */
+    float cell_factor;    /* supercell edge >= cell_factor * H: >= 1 takes the 27-supercell stencil;
+                             [1/2, 1) the 125-supercell stencil (5^3 cells of edge >= H/2: about 58% of
+                             the candidates, pair_balance 0 only); <= 0 means 1, below 1/2 means 1/2 */
     int32_t reserved;
 } sph_run_config;
 
@@ -165,12 +167,22 @@ int sph_ctx_destroy(sph_ctx* ctx);
 int sph_pool_create(int64_t max_particles, int64_t max_cells, int32_t max_runs, int32_t precision, int32_t mhd,
                     void* stream, sph_ctx** out);
 /* New run set = the kept runs (old indices, in the given order; their state and dt/time move with
- * them) followed by the added runs, to be filled with sph_upload. An added run's first step is its
- * priming step (see sph_upload); kept runs step on undisturbed. */
+ * them) followed by the added runs, to be filled with sph_upload or sph_init_run. An added run's
+ * first step is its priming step (see sph_upload); kept runs step on undisturbed. Every capacity is
+ * checked on the host first (a rejected call changes nothing); the change itself is enqueued on the
+ * context's stream (tables through pinned staging, state and per-run scalars moved by kernels), so
+ * the host does not wait for the device. */
 int sph_pool_set_runs(sph_ctx* ctx, const int32_t* keep, int32_t n_keep, const sph_run_config* add,
                       const int64_t* add_sizes, int32_t n_add);
 const char* sph_last_error(const sph_ctx* ctx);
 int sph_sync(sph_ctx* ctx);
+/* Marks on the context's stream, for hosts that keep work in flight (a serving loop that enqueues
+ * the next step before the last one has finished): sph_mark records the current point of the stream
+ * and returns its ticket (1, 2, ...); sph_mark_wait blocks until the stream has passed the point of
+ * that ticket (every earlier point too). Waiting on a ticket more than 64 marks old waits for a
+ * later point. The analogue of waiting on one step's sender rather than the whole queue. */
+int sph_mark(sph_ctx* ctx, uint64_t* ticket);
+int sph_mark_wait(sph_ctx* ctx, uint64_t ticket);
 int sph_get_grid(sph_ctx* ctx, int32_t run, sph_grid_info* out);
 /* Configuration and particle count of run `run` (either pointer may be NULL). */
 int sph_get_run(const sph_ctx* ctx, int32_t run, sph_run_config* cfg_out, int64_t* n_out);
@@ -219,7 +231,7 @@ typedef struct sph_lattice_block {
 int64_t sph_lattice_sites(const sph_lattice_block* blocks, int32_t n_blocks);
 /* Fill run `run` (whose size must equal the blocks' sites) on the device: ids 0..n-1 in block
  * order, smoothing length h0 of the run, then the setup's fields. Like an upload, the run restarts
- * at t = 0 and its next step primes it. */
+ * at t = 0 and its next step primes it. Enqueued on the context's stream; the host does not wait. */
 int sph_init_run(sph_ctx* ctx, int32_t run, const sph_lattice_block* blocks, int32_t n_blocks, uint64_t seed,
                  double jitter);
 
@@ -234,9 +246,9 @@ int sph_download(sph_ctx* ctx, int32_t run, sph_host_soa* dst);
  * out for output, spmacc/particles/algorithms/CopyParticlesToDynSoA.hpp, include/spearhed/memory.hpp:42-49):
  * enqueue device-to-host copies of the non-NULL columns of `dst` (and, if `cons` is non-NULL, the
  * run's FP64 sums and next dt as its last step left them) on the context's stream and return. The
- * host buffers are valid after the next call that synchronises the stream (sph_sync,
- * sph_pool_set_runs, sph_download); they should be pinned (sph_host_alloc) for the copies to run
- * asynchronously. Positions are available in mixed-precision contexts only. */
+ * host buffers are valid after the next call that synchronises the stream (sph_sync, sph_download,
+ * sph_upload); they should be pinned (sph_host_alloc) for the copies to run asynchronously.
+ * Positions are available in mixed-precision contexts only. */
 int sph_download_async(sph_ctx* ctx, int32_t run, const sph_host_soa* dst, sph_conserved* cons);
 /* Pinned host memory for sph_download_async / sph_upload staging. */
 int sph_host_alloc(int64_t bytes, void** out);

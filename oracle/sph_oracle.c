@@ -32,6 +32,14 @@ int orc_thread_num(void)
 #endif
 }
 
+/* Effective supercell factor: <= 0 (unset) is 1; below 1/2 it is 1/2. Edges of at least H reach the
+ * 27-cell stencil, edges of at least H/2 the 125-cell stencil. */
+static double cell_factor_of(float cf)
+{
+    if (!(cf > 0.0f)) return 1.0;
+    return cf < 0.5f ? 0.5 : (double)cf;
+}
+
 /* Uniform supercell grid: per axis the largest cell count whose edge is >= cell_factor * H with a
  * 1e-4 relative margin (so FP32 rounding of the cell index can never drop a pair within H).
  * This replaces the reference's irregular region AABBs and their brute-force overlap search
@@ -45,7 +53,8 @@ int orc_make_grid(const sph_run_config* cfg, orc_grid* g)
     g->H2 = g->H * g->H;
     g->Hd = (double)g->support * (double)cfg->h0;
     g->H2d = g->Hd * g->Hd;
-    double f = cfg->cell_factor > 1.0f ? (double)cfg->cell_factor : 1.0;
+    const double f = cell_factor_of(cfg->cell_factor);
+    g->reach = f < 1.0 ? 2 : 1;
     int64_t ncell = 1;
     for (int a = 0; a < 3; ++a) {
         double L = cfg->box_len[a];

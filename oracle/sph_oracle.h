@@ -39,6 +39,8 @@ typedef struct orc_grid {
     float lo32[3], L32[3], s32[3], inv_s32[3];
     float H, H2;
     double Hd, H2d;
+    int32_t reach;  /* supercells the stencil reaches per side: 1 (edge >= H, 27 cells) or 2 (edge >= H/2,
+                       125 cells) */
 } orc_grid;
 
 /* Particle state of one run. REAL arrays are void* so one struct serves both precisions:

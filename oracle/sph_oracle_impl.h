@@ -171,8 +171,8 @@ static inline int FN(nb_cell)(const orc_grid* g, int32_t prec, const sph_run_con
         int x = c[a] + d[a];
         int wrap = 0;
         if (cfg->periodic[a]) {
-            if (x < 0) wrap = -1;
-            else if (x >= g->n[a]) wrap = 1;
+            /* floor division: a stencil reaching 2 cells may wrap a 1-cell axis twice */
+            wrap = x >= 0 ? x / g->n[a] : -((-x + g->n[a] - 1) / g->n[a]);
             x -= wrap * g->n[a];
         } else if (x < 0 || x >= g->n[a]) {
             return 0;
@@ -211,16 +211,18 @@ typedef struct FN(span) {
     REAL sh[3];
 } FN(span);
 
-/* The 27-supercell stencil of one cell as spans (fewer at a non-periodic wall). */
+/* The stencil of one cell as spans, (2 reach + 1)^3 supercells x fastest (fewer at a non-periodic
+ * wall): 27 for supercells of edge >= H, 125 for edge >= H/2. */
 static int FN(cell_spans)(const orc_grid* g, int32_t prec, const sph_run_config* cfg, const uint32_t* cell_start,
                           int cell, FN(span)* sp)
 {
     int c[3];
     cell_coords(g, cell, c);
     int n = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
+    const int r = g->reach;
+    for (int dz = -r; dz <= r; ++dz)
+        for (int dy = -r; dy <= r; ++dy)
+            for (int dx = -r; dx <= r; ++dx) {
                 int d[3] = {dx, dy, dz};
                 int cc;
                 if (FN(nb_cell)(g, prec, cfg, c, d, &cc, sp[n].sh)) {
@@ -381,7 +383,7 @@ static double FN(clean_speed)(const sph_run_config* cfg, const orc_state* st)
     return chmax;
 }
 
-/* Density summation over the 27-supercell stencil (DensitySummation.hpp:50-138 semantics). */
+/* Density summation over the supercell stencil (DensitySummation.hpp:50-138 semantics). */
 int FN(orc_density)(const sph_run_config* cfg, int32_t prec, orc_state* st, const uint32_t* cell_start)
 {
     orc_grid g;
@@ -395,7 +397,7 @@ int FN(orc_density)(const sph_run_config* cfg, int32_t prec, orc_state* st, cons
 
 #pragma omp parallel for schedule(dynamic, 16)
     for (int cell = 0; cell < g.ncell; ++cell) {
-        FN(span) sp[27];
+        FN(span) sp[125];
         const int nsp = FN(cell_spans)(&g, prec, cfg, cell_start, cell, sp);
         for (int64_t i = cell_start[cell]; i < (int64_t)cell_start[cell + 1]; ++i) {
             if (!is_target(st->species[i], st->mask[i])) {
@@ -594,7 +596,7 @@ static void FN(force_one)(const FN(fctx)* F, orc_state* st, int64_t i, const FN(
     }
 }
 
-/* Force pass over the 27-supercell stencil. */
+/* Force pass over the supercell stencil. */
 int FN(orc_forces)(const sph_run_config* cfg, int32_t prec, orc_state* st, const uint32_t* cell_start)
 {
     orc_grid g;
@@ -610,7 +612,7 @@ int FN(orc_forces)(const sph_run_config* cfg, int32_t prec, orc_state* st, const
 
 #pragma omp parallel for schedule(dynamic, 16)
     for (int cell = 0; cell < g.ncell; ++cell) {
-        FN(span) sp[27];
+        FN(span) sp[125];
         const int nsp = FN(cell_spans)(&g, prec, cfg, cell_start, cell, sp);
         for (int64_t i = cell_start[cell]; i < (int64_t)cell_start[cell + 1]; ++i) {
             if (!is_target(st->species[i], st->mask[i])) continue;

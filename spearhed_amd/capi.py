@@ -205,6 +205,8 @@ PROTOTYPES = {
                                     C.POINTER(C.c_int64), C.c_int32]),
     "sph_last_error": (C.c_char_p, [_CTX]),
     "sph_sync": (C.c_int, [_CTX]),
+    "sph_mark": (C.c_int, [_CTX, C.POINTER(C.c_uint64)]),
+    "sph_mark_wait": (C.c_int, [_CTX, C.c_uint64]),
     "sph_get_grid": (C.c_int, [_CTX, C.c_int32, C.POINTER(GridInfo)]),
     "sph_get_run": (C.c_int, [_CTX, C.c_int32, C.POINTER(RunConfig), C.POINTER(C.c_int64)]),
     "sph_run_count": (C.c_int32, [_CTX]),

@@ -44,6 +44,16 @@ struct sph_ctx {
     sph_tuning tune = {128, 64, 0, 64, 0};  // fastest measured on the 2M-particle MHD turbulence step
     void* arena = nullptr;
     size_t arena_bytes = 0;
+    // pinned staging of the run tables, two halves used in turn (an event marks when the copies out
+    // of a half have run), so that a change of the run set is enqueued without waiting on the host
+    char* stage = nullptr;
+    size_t stage_half = 0;
+    int stage_next = 0;
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    // marks on the stream (sph_mark / sph_mark_wait): a ring of events, ticket t in slot t % kMarks
+    static constexpr int kMarks = 64;
+    hipEvent_t marks[kMarks] = {};
+    uint64_t next_mark = 1;
     bool binned = false;   // keys + histogram valid for the current positions
     bool sorted = false;   // buffers sorted by (cell, id) for the current positions
     bool dens_valid = false;
@@ -88,6 +98,14 @@ float sigma_of(int kernel)
     return 21.0f / (16.0f * pi);
 }
 
+// Effective supercell factor: <= 0 (unset) is 1; below 1/2 it is 1/2. Edges of at least H take the
+// 27-supercell stencil, edges of at least H/2 the 125-supercell stencil (reach 2).
+double cell_factor_of(float cf)
+{
+    if (!(cf > 0.0f)) return 1.0;
+    return cf < 0.5f ? 0.5 : (double)cf;
+}
+
 // Supercell grid of one run: per axis the largest count whose edge is >= cell_factor*H*(1+1e-4).
 bool make_run(const sph_run_config& c, RunDev& R, std::string& why)
 {
@@ -104,7 +122,8 @@ bool make_run(const sph_run_config& c, RunDev& R, std::string& why)
     R.support = support_of(c.kernel);
     R.H = (float)R.support * c.h0;
     R.H2 = R.H * R.H;
-    const double f = c.cell_factor > 1.0f ? (double)c.cell_factor : 1.0;
+    const double f = cell_factor_of(c.cell_factor);
+    R.reach = f < 1.0 ? 2 : 1;
     int64_t nc = 1;
     for (int a = 0; a < 3; ++a) {
         const double L = c.box_len[a];
@@ -214,6 +233,17 @@ void carve(Carver& cv, sph_ctx* c, int64_t N, int ncell_cap, int nkb, int nruns)
     W.blk_run = cv.take<int32_t>(nkb);
     W.prime_flag = cv.take<int32_t>(nruns);
     W.move_tab = cv.take<int64_t>((int64_t)nruns * 3);
+    W.run_src = cv.take<int32_t>(nruns);
+    W.run_tmp = cv.take<double>((int64_t)nruns * 3);
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Bytes of one staging half: run table, block -> run table, run sources, move table.
+size_t stage_half_bytes(int cap_r, int cap_kb)
+{
+    return align256(sizeof(RunDev) * (size_t)cap_r) + align256(4 * (size_t)cap_kb) + align256(4 * (size_t)cap_r) +
+           align256(24 * (size_t)cap_r);
 }
 
 LaunchCtx lctx(sph_ctx* c)
@@ -233,6 +263,8 @@ LaunchCtx lctx(sph_ctx* c)
     L.kern = c->nruns > 0 ? c->cfg[0].kernel : -1;
     for (int r = 1; r < c->nruns; ++r)
         if (c->cfg[r].kernel != L.kern) L.kern = -1;
+    L.rmax = 1;
+    for (const RunDev& R : c->runs_h) L.rmax = std::max(L.rmax, (int)R.reach);
     L.persistent_waves = 0;
     if (c->tune.waves_per_cu > 0) {
         const int64_t w = (int64_t)c->tune.waves_per_cu * c->num_cu;
@@ -358,16 +390,70 @@ int reserve(sph_ctx* c, int64_t cap_p, int64_t cap_c, int cap_r, int cap_kb)
     HIPTRY(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->arena), 0u, (int64_t)(c->arena_bytes / 4), c->stream));
     HIPTRY(c, hipStreamSynchronize(c->stream));
     HIPTRY(c, hipMalloc(&c->runs_d, sizeof(RunDev) * cap_r));
+    c->stage_half = stage_half_bytes(cap_r, cap_kb);
+    HIPTRY(c, hipHostMalloc((void**)&c->stage, 2 * c->stage_half, hipHostMallocDefault));
+    for (auto& e : c->stage_ev) {
+        HIPTRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPTRY(c, hipEventRecord(e, c->stream));
+    }
     return SPH_OK;
 }
 
-// Lay runs out contiguously (particles, supercells, kick blocks) and upload the run and block
-// tables with the given per-run dt and time; runs flagged in `fresh` are marked for priming.
+// Upload the run table (and, when given, the block -> run table, the run sources and the move table)
+// through the next staging half, on the context's stream. The host waits only until the copies out
+// of that half from two uploads ago have run.
+int stage_tables(sph_ctx* c, const std::vector<RunDev>& runs, int kb, const std::vector<int32_t>* src,
+                 const std::vector<int64_t>* move)
+{
+    const int h = c->stage_next;
+    c->stage_next ^= 1;
+    HIPTRY(c, hipEventSynchronize(c->stage_ev[h]));
+    char* base = c->stage + (size_t)h * c->stage_half;
+    const size_t cap_r = (size_t)c->cap_runs, cap_kb = (size_t)c->cap_kb;
+    RunDev* t_runs = reinterpret_cast<RunDev*>(base);
+    int32_t* t_blk = reinterpret_cast<int32_t*>(base + align256(sizeof(RunDev) * cap_r));
+    int32_t* t_src = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(t_blk) + align256(4 * cap_kb));
+    int64_t* t_move = reinterpret_cast<int64_t*>(reinterpret_cast<char*>(t_src) + align256(4 * cap_r));
+    const size_t nr = runs.size();
+    if (nr > 0) {
+        std::copy(runs.begin(), runs.end(), t_runs);
+        HIPTRY(c, hipMemcpyAsync(c->runs_d, t_runs, sizeof(RunDev) * nr, hipMemcpyHostToDevice, c->stream));
+    }
+    if (src && nr > 0) {
+        for (size_t r = 0; r < nr; ++r)
+            for (int b = 0; b < runs[r].kb_count; ++b) t_blk[runs[r].kb_first + b] = (int32_t)r;
+        HIPTRY(c, hipMemcpyAsync(c->W.blk_run, t_blk, sizeof(int32_t) * kb, hipMemcpyHostToDevice, c->stream));
+        std::copy(src->begin(), src->end(), t_src);
+        HIPTRY(c, hipMemcpyAsync(c->W.run_src, t_src, sizeof(int32_t) * nr, hipMemcpyHostToDevice, c->stream));
+    }
+    if (move && !move->empty()) {
+        std::copy(move->begin(), move->end(), t_move);
+        HIPTRY(c, hipMemcpyAsync(c->W.move_tab, t_move, sizeof(int64_t) * move->size(), hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    HIPTRY(c, hipEventRecord(c->stage_ev[h], c->stream));
+    return SPH_OK;
+}
+
+// Lay runs out contiguously (particles, supercells, kick blocks) and enqueue the run and block tables:
+// new run r continues old run src[r] (its dt, time and priming flag) or, for src[r] < 0, starts fresh
+// and is primed by its next step. Nothing waits on the device.
+// Supercells finer than H are swept by the chunked lane loop only (pair_balance 0).
+bool balance_allowed(const std::vector<sph_run_config>& cfg, int pair_balance)
+{
+    if (pair_balance == 0) return true;
+    for (const sph_run_config& r : cfg)
+        if (cell_factor_of(r.cell_factor) < 1.0) return false;
+    return true;
+}
+const char* kFineBalance = "supercells finer than H (cell_factor < 1) need pair_balance 0";
+
 int layout(sph_ctx* c, const std::vector<sph_run_config>& cfg, const std::vector<int64_t>& sizes,
-           const std::vector<char>& fresh, const std::vector<double>& dt, const std::vector<double>& time)
+           const std::vector<int32_t>& src, const std::vector<int64_t>* move = nullptr)
 {
     const int nr = (int)cfg.size();
     if (nr > c->cap_runs) return fail(c, SPH_E_CAPACITY, "run table capacity exceeded");
+    if (!balance_allowed(cfg, c->tune.pair_balance)) return fail(c, SPH_E_INVALID, kFineBalance);
     std::vector<RunDev> runs(nr);
     int64_t p0 = 0;
     int64_t c0 = 0;
@@ -389,25 +475,16 @@ int layout(sph_ctx* c, const std::vector<sph_run_config>& cfg, const std::vector
     if (p0 > c->cap_particles) return fail(c, SPH_E_CAPACITY, "particle capacity exceeded");
     if (c0 > c->ncell_cap) return fail(c, SPH_E_CAPACITY, "supercell capacity exceeded");
     if (kb > c->cap_kb) return fail(c, SPH_E_CAPACITY, "block table capacity exceeded");
+    const int n_old = c->nruns;
     c->nruns = nr;
     c->cfg = cfg;
     c->runs_h = runs;
     c->ntot = p0;
     c->ncell_tot = (int)c0;
     c->nkb = kb;
-    if (nr > 0) {
-        HIPTRY(c, hipMemcpyAsync(c->runs_d, c->runs_h.data(), sizeof(RunDev) * nr, hipMemcpyHostToDevice, c->stream));
-        std::vector<int32_t> blk(kb);
-        for (int r = 0; r < nr; ++r)
-            for (int b = 0; b < runs[r].kb_count; ++b) blk[runs[r].kb_first + b] = r;
-        HIPTRY(c, hipMemcpyAsync(c->W.blk_run, blk.data(), sizeof(int32_t) * kb, hipMemcpyHostToDevice, c->stream));
-        std::vector<int32_t> flags(nr);
-        for (int r = 0; r < nr; ++r) flags[r] = fresh[r] ? 1 : 0;
-        HIPTRY(c, hipMemcpyAsync(c->W.prime_flag, flags.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice, c->stream));
-        HIPTRY(c, hipMemcpyAsync(c->W.dt, dt.data(), sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
-        HIPTRY(c, hipMemcpyAsync(c->W.time, time.data(), sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
-        HIPTRY(c, hipStreamSynchronize(c->stream));
-    }
+    int rc = stage_tables(c, runs, kb, &src, move);
+    if (rc) return rc;
+    HIPTRY(c, launch_remap_runs(c->runs_d, c->W, n_old, nr, c->stream));
     HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_cap, c->stream));
     c->binned = false;
     c->sorted = false;
@@ -419,6 +496,7 @@ int layout(sph_ctx* c, const std::vector<sph_run_config>& cfg, const std::vector
 // Re-derive every run's grid (after a cell_factor change) keeping particle offsets.
 int regrid(sph_ctx* c)
 {
+    if (!balance_allowed(c->cfg, c->tune.pair_balance)) return fail(c, SPH_E_INVALID, kFineBalance);
     std::vector<RunDev> runs(c->runs_h);  // committed only if every run fits
     int64_t c0 = 0;
     for (int r = 0; r < c->nruns; ++r) {
@@ -436,7 +514,8 @@ int regrid(sph_ctx* c)
     }
     c->runs_h = runs;
     c->ncell_tot = (int)c0;
-    HIPTRY(c, hipMemcpyAsync(c->runs_d, c->runs_h.data(), sizeof(RunDev) * c->nruns, hipMemcpyHostToDevice, c->stream));
+    int rc = stage_tables(c, c->runs_h, c->nkb, nullptr, nullptr);
+    if (rc) return rc;
     HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_cap, c->stream));
     c->binned = false;
     c->sorted = false;
@@ -528,10 +607,7 @@ int sph_ctx_create(const sph_run_config* runs, const int64_t* capacity, int32_t 
     int rc = open_stream(c, stream);
     if (!rc) rc = reserve(c, cap_p, cap_c, n_runs, cap_kb);
     if (rc) return rc;
-    std::vector<char> fresh(n_runs, 1);
-    std::vector<double> dt(n_runs), time(n_runs, 0.0);
-    for (int r = 0; r < n_runs; ++r) dt[r] = cfg[r].dt_fixed > 0 ? cfg[r].dt_fixed : 0.0;
-    return layout(c, cfg, sizes, fresh, dt, time);
+    return layout(c, cfg, sizes, std::vector<int32_t>(n_runs, -1));
 }
 
 int sph_pool_create(int64_t max_particles, int64_t max_cells, int32_t max_runs, int32_t precision, int32_t mhd,
@@ -556,7 +632,6 @@ int sph_pool_set_runs(sph_ctx* c, const int32_t* keep, int32_t n_keep, const sph
                       const int64_t* add_sizes, int32_t n_add)
 {
     if (!c || n_keep < 0 || n_add < 0 || (n_keep && !keep) || (n_add && (!add || !add_sizes))) return SPH_E_INVALID;
-    HIPTRY(c, hipStreamSynchronize(c->stream));
     std::vector<char> seen(c->nruns, 0);
     for (int k = 0; k < n_keep; ++k) {
         if (keep[k] < 0 || keep[k] >= c->nruns || seen[keep[k]])
@@ -591,30 +666,14 @@ int sph_pool_set_runs(sph_ctx* c, const int32_t* keep, int32_t n_keep, const sph
     // every capacity is checked before anything moves: a rejected call leaves the pool as it was
     if (parts > c->cap_particles || cells > c->ncell_cap || (int)cfg.size() > c->cap_runs || blocks > c->cap_kb)
         return fail(c, SPH_E_CAPACITY, "runs do not fit the pool's capacity");
-    // per-run scalars of the kept runs (a kept run not yet stepped still primes on its first step)
-    std::vector<double> old_dt(c->nruns), old_time(c->nruns);
-    std::vector<int32_t> old_prime(c->nruns);
-    if (c->nruns) {
-        HIPTRY(c, hipMemcpyAsync(old_dt.data(), c->W.dt, sizeof(double) * c->nruns, hipMemcpyDeviceToHost, c->stream));
-        HIPTRY(c, hipMemcpyAsync(old_time.data(), c->W.time, sizeof(double) * c->nruns, hipMemcpyDeviceToHost, c->stream));
-        HIPTRY(c, hipMemcpyAsync(old_prime.data(), c->W.prime_flag, sizeof(int32_t) * c->nruns, hipMemcpyDeviceToHost,
-                                 c->stream));
-        HIPTRY(c, hipStreamSynchronize(c->stream));
-    }
-    std::vector<double> dt, time;
-    std::vector<char> fresh;
-    for (int k = 0; k < n_keep; ++k) {
-        dt.push_back(old_dt[keep[k]]);
-        time.push_back(old_time[keep[k]]);
-        fresh.push_back(old_prime[keep[k]] ? 1 : 0);
-    }
-    for (int a = 0; a < n_add; ++a) {
-        dt.push_back(add[a].dt_fixed > 0 ? add[a].dt_fixed : 0.0);
-        time.push_back(0.0);
-        fresh.push_back(1);
-    }
-    // move the kept runs' state into the other buffer set at their new offsets: one table upload
-    // and one kernel for all runs and columns
+    if (!balance_allowed(cfg, c->tune.pair_balance)) return fail(c, SPH_E_INVALID, kFineBalance);
+    // new run k < n_keep continues old run keep[k] (its dt, time and priming flag move with it on the
+    // device: a kept run not yet stepped still primes on its first step); added runs start fresh
+    std::vector<int32_t> src;
+    for (int k = 0; k < n_keep; ++k) src.push_back(keep[k]);
+    for (int a = 0; a < n_add; ++a) src.push_back(-1);
+    // the kept runs' state moves into the other buffer set at their new offsets: one table and one
+    // kernel for all runs and columns
     std::vector<int64_t> tab;
     int64_t dst = 0;
     for (int k = 0; k < n_keep; ++k) {
@@ -623,15 +682,14 @@ int sph_pool_set_runs(sph_ctx* c, const int32_t* keep, int32_t n_keep, const sph
         tab.push_back(c->runs_h[keep[k]].np);
         dst += c->runs_h[keep[k]].np;
     }
-    if (n_keep > 0) {
-        HIPTRY(c, hipMemcpyAsync(c->W.move_tab, tab.data(), sizeof(int64_t) * tab.size(), hipMemcpyHostToDevice,
-                                 c->stream));
-        HIPTRY(c, launch_move_runs(c->S[c->cur], c->S[c->cur ^ 1], c->W.move_tab, n_keep, dst, c->mixed ? 1 : 0,
+    const int from = c->cur;
+    int rc = layout(c, cfg, sizes, src, &tab);  // enqueues the tables, including the move table
+    if (rc) return rc;
+    if (n_keep > 0)
+        HIPTRY(c, launch_move_runs(c->S[from], c->S[from ^ 1], c->W.move_tab, n_keep, dst, c->mixed ? 1 : 0,
                                    c->has_mhd ? 1 : 0, c->stream));
-        HIPTRY(c, hipStreamSynchronize(c->stream));  // `tab` is host memory read by the copy
-    }
-    c->cur ^= 1;
-    return layout(c, cfg, sizes, fresh, dt, time);
+    c->cur = from ^ 1;
+    return SPH_OK;
 }
 
 int sph_ctx_destroy(sph_ctx* c)
@@ -640,6 +698,11 @@ int sph_ctx_destroy(sph_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     invalidate_graph(c);
     for (auto e : c->prof_ev) hipEventDestroy(e);
+    for (auto e : c->stage_ev)
+        if (e) hipEventDestroy(e);
+    for (auto e : c->marks)
+        if (e) hipEventDestroy(e);
+    if (c->stage) hipHostFree(c->stage);
     if (c->arena) hipFree(c->arena);
     if (c->runs_d) hipFree(c->runs_d);
     if (c->own_stream) hipStreamDestroy(c->stream);
@@ -648,6 +711,26 @@ int sph_ctx_destroy(sph_ctx* c)
 }
 
 const char* sph_last_error(const sph_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int sph_mark(sph_ctx* c, uint64_t* ticket)
+{
+    if (!c || !ticket) return SPH_E_INVALID;
+    const uint64_t t = c->next_mark;
+    hipEvent_t& e = c->marks[t % sph_ctx::kMarks];
+    if (!e) HIPTRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPTRY(c, hipEventRecord(e, c->stream));
+    ++c->next_mark;
+    *ticket = t;
+    return SPH_OK;
+}
+
+int sph_mark_wait(sph_ctx* c, uint64_t ticket)
+{
+    if (!c || ticket == 0 || ticket >= c->next_mark) return SPH_E_INVALID;
+    // a slot reused by a later mark waits for that later point: the stream has then passed both
+    HIPTRY(c, hipEventSynchronize(c->marks[ticket % sph_ctx::kMarks]));
+    return SPH_OK;
+}
 
 int sph_sync(sph_ctx* c)
 {
@@ -697,6 +780,7 @@ int sph_set_tuning(sph_ctx* c, const sph_tuning* t)
         return fail(c, SPH_E_INVALID, "max_lanes_per_target must be in [1, 64]");
     if (t->waves_per_cu != 0 && (t->waves_per_cu < 1 || t->waves_per_cu > 64))
         return fail(c, SPH_E_INVALID, "waves_per_cu must be 0 (one wave per supercell) or in [1, 64]");
+    if (!balance_allowed(c->cfg, t->pair_balance)) return fail(c, SPH_E_INVALID, kFineBalance);
     c->tune = *t;
     c->dens_valid = false;  // the force pass replays masks laid out for the density pass's tuning
     invalidate_graph(c);
@@ -716,17 +800,15 @@ int64_t sph_bytes_per_run(const sph_run_config* cfg, int64_t n, int32_t precisio
     RunDev R;
     std::string why;
     if (!make_run(*cfg, R, why)) return -1;  // the configuration as given must be valid
-    sph_run_config fine = *cfg;
-    fine.cell_factor = 1.0f;
-    if (!make_run(fine, R, why)) return -1;
+    const int64_t ncell = cells_needed(*cfg, why);
     const int64_t pos = precision == SPH_PREC_MIXED ? 24 : 12;
     const int64_t mhd = cfg->mhd ? 1 : 0;
     const int64_t state = 8 + 2 + pos + 12 + 16 + mhd * 16;
     const int64_t work = 12 + 4 + 4 + 4 + 12 + 8 + 64 + mhd * (12 + 4 + 4 + 16);
     const int64_t blocks = (n + 255) / 256;  // FP64 partial sums (6 doubles) + block->run entry per block
-    const int64_t per_run = 8 + 8 + 4 + 8 * 8 + 4 + 3 * 8;  // dt, time, cleaning speed, sums, flag, move entry
-    return n * (2 * state + work) + blocks * (6 * 8 + 4) + (int64_t)R.ncell * (12 + 4 * kMaskWords * 64) + per_run +
-           4096;
+    // dt, time, cleaning speed, sums, flag, move entry, source index, saved scalars
+    const int64_t per_run = 8 + 8 + 4 + 8 * 8 + 4 + 3 * 8 + 4 + 3 * 8;
+    return n * (2 * state + work) + blocks * (6 * 8 + 4) + ncell * (12 + 4 * kMaskWords * 64) + per_run + 4096;
 }
 
 int64_t sph_pool_bytes(int64_t max_particles, int64_t max_cells, int32_t max_runs, int32_t precision, int32_t mhd)
@@ -808,12 +890,8 @@ int sph_upload(sph_ctx* c, int32_t run, const sph_host_soa* s)
     if (c->has_mhd) rc |= copy_col<float>(c, S.psi + o, s->psi, n, 0.0f);
     if (rc) return SPH_E_DEVICE;
     // new initial data: the run restarts at t = 0 and takes its first dt from the next prime
-    const double dt0 = c->cfg[run].dt_fixed > 0 ? c->cfg[run].dt_fixed : 0.0, t0 = 0.0;
-    const int32_t one = 1;
-    HIPTRY(c, hipMemcpyAsync(c->W.dt + run, &dt0, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPTRY(c, hipMemcpyAsync(c->W.time + run, &t0, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPTRY(c, hipMemcpyAsync(c->W.prime_flag + run, &one, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-    HIPTRY(c, hipStreamSynchronize(c->stream));
+    HIPTRY(c, launch_restart_run(c->runs_d, c->W, run, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));  // the columns above were copied from pageable memory
     if (c->binned) HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_tot, c->stream));
     c->binned = false;
     c->sorted = false;
@@ -844,13 +922,8 @@ int sph_init_run(sph_ctx* c, int32_t run, const sph_lattice_block* blocks, int32
     const sph_run_config& cfg = c->cfg[run];
     HIPTRY(c, launch_init_lattice(blocks, n_blocks, R.np, R.p0, seed, jitter, cfg.h0, cfg.gamma, c->mixed ? 1 : 0,
                                   c->has_mhd ? 1 : 0, c->S[c->cur], c->stream));
-    // as an upload: time restarts at 0 and the next step primes the run
-    const double dt0 = cfg.dt_fixed > 0 ? cfg.dt_fixed : 0.0, t0 = 0.0;
-    const int32_t one = 1;
-    HIPTRY(c, hipMemcpyAsync(c->W.dt + run, &dt0, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPTRY(c, hipMemcpyAsync(c->W.time + run, &t0, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPTRY(c, hipMemcpyAsync(c->W.prime_flag + run, &one, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-    HIPTRY(c, hipStreamSynchronize(c->stream));
+    // as an upload: time restarts at 0 and the next step primes the run (enqueued, nothing waits)
+    HIPTRY(c, launch_restart_run(c->runs_d, c->W, run, c->stream));
     if (c->binned) HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_tot, c->stream));
     c->binned = false;
     c->sorted = false;
@@ -1213,19 +1286,43 @@ int sph_autotune(sph_ctx* c, int32_t n_steps, sph_tuning* best)
             for (int k : {16, 64})
                 for (int bal : {0, 1}) cands.push_back({dc, fc, 0, k, bal});
     for (int k : {16, 64}) cands.push_back({512, 128, 0, k, 2});
-    const float cfs[2] = {1.0f, 1.15f};
+    // supercell edges of 1/2, 1 and 1.15 interaction radii, each no finer than the run's own setting;
+    // a grid the context's supercell table cannot hold is skipped (a context reserves cells for its
+    // runs' own grids and for edges of 1 H)
+    std::vector<float> cfs;
+    for (float cf : {0.5f, 1.0f, 1.15f}) {
+        std::vector<sph_run_config> cfg = cfg0;
+        int64_t cells = 0;
+        for (auto& r : cfg) {
+            r.cell_factor = (float)std::max<double>(cf, cell_factor_of(r.cell_factor));
+            std::string why;
+            RunDev R;
+            cells += make_run(r, R, why) ? R.ncell : (1LL << 40);
+        }
+        bool dup = false;
+        for (float prev : cfs) {
+            bool same = true;
+            for (const auto& r : cfg0)
+                same = same && std::max<double>(prev, cell_factor_of(r.cell_factor)) ==
+                                   std::max<double>(cf, cell_factor_of(r.cell_factor));
+            dup = dup || same;
+        }
+        if (cells <= c->ncell_cap && !dup) cfs.push_back(cf);
+    }
     const int wpcs[4] = {4, 8, 16, 32};
 
     double best_t = 1e300;
     sph_tuning best_tune = t0;
     float best_cf = -1.0f;
     auto time_one = [&](float cf, const sph_tuning& cand) -> int {
-        for (int r = 0; r < c->nruns; ++r) c->cfg[r].cell_factor = std::max(cf, cfg0[r].cell_factor);
+        for (int r = 0; r < c->nruns; ++r)
+            c->cfg[r].cell_factor = (float)std::max<double>(cf, cell_factor_of(cfg0[r].cell_factor));
+        if (!balance_allowed(c->cfg, cand.pair_balance)) return SPH_OK;  // not a candidate on this grid
         HIPTRY(c, hipMemcpyAsync(c->arena, sc.backup, bytes, hipMemcpyDeviceToDevice, c->stream));
         c->cur = cur0;
+        c->tune = cand;
         int rc = regrid(c);
         if (rc) return rc;
-        c->tune = cand;
         if ((rc = do_step(c))) return rc;  // warm-up
         HIPTRY(c, hipEventRecord(sc.e0, c->stream));
         for (int s = 0; s < n_steps; ++s)
@@ -1260,7 +1357,8 @@ int sph_autotune(sph_ctx* c, int32_t n_steps, sph_tuning* best)
     // restore the state; apply the winner only if the search completed
     c->cfg = cfg0;
     if (!rc && best_cf > 0)
-        for (int r = 0; r < c->nruns; ++r) c->cfg[r].cell_factor = std::max(best_cf, cfg0[r].cell_factor);
+        for (int r = 0; r < c->nruns; ++r)
+            c->cfg[r].cell_factor = (float)std::max<double>(best_cf, cell_factor_of(cfg0[r].cell_factor));
     c->tune = rc ? t0 : best_tune;
     c->cur = cur0;
     HIPTRY(c, hipMemcpyAsync(c->arena, sc.backup, bytes, hipMemcpyDeviceToDevice, c->stream));

@@ -36,7 +36,7 @@ struct RunDev {
     double dt_fixed;
     int32_t kb_first, kb_count;  // particle-kernel blocks of this run
     float f0;                    // kernel shape at q = 0
-    int32_t pad;
+    int32_t reach;               // supercells the stencil reaches per side: 1 (edge >= H) or 2 (edge >= H/2)
 };
 
 // Device arrays of one particle buffer set.
@@ -85,6 +85,8 @@ struct WorkBuf {
     int32_t* blk_run; // per particle-kernel block: run id
     int32_t* prime_flag; // per run: 1 = the next non-advancing reduction sets the run's first dt
     int64_t* move_tab;   // per kept run when a pool's run set changes: destination start, source start, count
+    int32_t* run_src;    // per run of a new run set: the old run it continues, or -1 for a new run
+    double* run_tmp;     // per old run: its dt, time and priming flag, saved while the run set changes
 };
 
 constexpr float kSentinel = 8.507059e37f;  // FLT_MAX / 4: sentinel^2 overflows to +inf, so r2 < H2 fails

@@ -291,28 +291,48 @@ __global__ __launch_bounds__(256) void k_gather(const RunDev* __restrict__ runs,
 }
 
 // ------------------------------------------------------------------------------------------------
-// shared set-up of the 27-supercell virtual neighbour list (one wave)
+// shared set-up of the supercell stencil's virtual neighbour list (one wave): the (2 reach + 1)^3
+// supercells around a cell, x fastest, concatenated. NS = 27 serves supercells of edge >= H (reach
+// 1), NS = 125 also those of edge >= H/2 (reach 2: 5^3 stencil, about 58% of the candidates).
 // ------------------------------------------------------------------------------------------------
-struct NbList {
-    int start[27];
-    int off[28];
-    float sh[27][3];
+template <int NS>
+struct NbListT {
+    int start[NS];
+    int off[NS + 1];  // entries past the run's stencil hold the total, so no search selects them
+    float sh[NS][3];
 };
+template <int RMAX>
+using NbListR = NbListT<RMAX == 1 ? 27 : 125>;
+// LDS candidate slot indices of the balanced pair loop
+template <int CH>
+using ListT = typename std::conditional<(CH <= 256), uint8_t, uint16_t>::type;
 
-// Neighbour supercell `seg` (0..26, x fastest, d = seg%3-1, seg/3%3-1, seg/9-1) of cell c: returns
-// false outside a non-periodic axis; else the global cell and the staged-position shift (the
-// periodic image for FP32 absolute coordinates, d * cell edge for supercell-relative ones).
-template <bool MIXED>
+// Neighbour supercell `seg` of cell c in a stencil of side 2 reach + 1 (x fastest, offsets
+// d = seg % side - reach, ...): returns false outside a non-periodic axis; else the global cell and
+// the staged-position shift (the periodic image for FP32 absolute coordinates, d * cell edge for
+// supercell-relative ones). A 1-cell periodic axis is wrapped twice by a reach of 2.
+template <bool MIXED, int RMAX>
 __device__ __forceinline__ bool nb_cell(const RunDev& R, const int c[3], int seg, int& cc, float sh[3])
 {
-    const int d[3] = {seg % 3 - 1, (seg / 3) % 3 - 1, seg / 9 - 1};
+    int d[3];
+    if (RMAX == 1) {
+        d[0] = seg % 3 - 1;
+        d[1] = (seg / 3) % 3 - 1;
+        d[2] = seg / 9 - 1;
+    } else {
+        const int r = R.reach, side = 2 * R.reach + 1;
+        d[0] = seg % side - r;
+        d[1] = (seg / side) % side - r;
+        d[2] = seg / (side * side) - r;
+    }
     int idx[3];
     bool ok = true;
     for (int a = 0; a < 3; ++a) {
         int x = c[a] + d[a];
         int wrap = 0;
         if (R.periodic[a]) {
-            wrap = x < 0 ? -1 : (x >= R.n[a] ? 1 : 0);
+            if (RMAX == 1) wrap = x < 0 ? -1 : (x >= R.n[a] ? 1 : 0);
+            else wrap = x >= 0 ? x / R.n[a] : -((R.n[a] - 1 - x) / R.n[a]);
             x -= wrap * R.n[a];
         } else if (x < 0 || x >= R.n[a]) {
             ok = false;
@@ -324,40 +344,61 @@ __device__ __forceinline__ bool nb_cell(const RunDev& R, const int c[3], int seg
     return ok;
 }
 
-template <bool MIXED>
-__device__ __forceinline__ void build_nblist(const RunDev& R, int lcell, const uint32_t* __restrict__ cs, NbList& L)
+// Index of the cell's own supercell in its stencil (the centre).
+template <int RMAX>
+__device__ __forceinline__ int self_segment(const RunDev& R)
 {
+    if (RMAX == 1) return 13;
+    const int side = 2 * R.reach + 1;
+    return (side * side * side - 1) / 2;
+}
+
+template <bool MIXED, int RMAX>
+__device__ __forceinline__ void build_nblist(const RunDev& R, int lcell, const uint32_t* __restrict__ cs,
+                                             NbListR<RMAX>& L)
+{
+    constexpr int NS = RMAX == 1 ? 27 : 125;
     const int lane = threadIdx.x;
     const int c[3] = {lcell % R.n[0], (lcell / R.n[0]) % R.n[1], lcell / (R.n[0] * R.n[1])};
-    int cnt = 0, st = 0;
-    float sh[3] = {0.f, 0.f, 0.f};
-    if (lane < 27) {
-        int cc;
-        if (nb_cell<MIXED>(R, c, lane, cc, sh)) {
-            st = (int)cs[cc];
-            cnt = (int)cs[cc + 1] - st;
+    const int side = RMAX == 1 ? 3 : 2 * R.reach + 1;
+    const int nseg = side * side * side;
+    int carry = 0;
+#pragma unroll
+    for (int base = 0; base < NS; base += 64) {
+        const int seg = base + lane;
+        int cnt = 0, st = 0;
+        float sh[3] = {0.f, 0.f, 0.f};
+        if (seg < nseg) {
+            int cc;
+            if (nb_cell<MIXED, RMAX>(R, c, seg, cc, sh)) {
+                st = (int)cs[cc];
+                cnt = (int)cs[cc + 1] - st;
+            }
         }
+        const int incl = wave_incl_scan(cnt) + carry;
+        if (seg < NS) {
+            L.start[seg] = st;
+            L.off[seg] = seg < nseg ? incl - cnt : 0;  // past the stencil: overwritten with the total below
+            L.sh[seg][0] = sh[0];
+            L.sh[seg][1] = sh[1];
+            L.sh[seg][2] = sh[2];
+        }
+        carry = __shfl(incl, 63, 64);
     }
-    int incl = wave_incl_scan(cnt);
-    if (lane < 27) {
-        L.start[lane] = st;
-        L.off[lane] = incl - cnt;
-        L.sh[lane][0] = sh[0];
-        L.sh[lane][1] = sh[1];
-        L.sh[lane][2] = sh[2];
-    }
-    if (lane == 26) L.off[27] = incl;
+    // the total after the last stencil entry, and in every entry past the run's stencil
+    for (int k = nseg + lane; k <= NS; k += 64) L.off[k] = carry;
     __syncthreads();
 }
 
-// segment of virtual index v (largest k with off[k] <= v, over non-decreasing off[0..26])
-__device__ __forceinline__ int nb_segment(const NbList& L, int v)
+// segment of virtual index v (largest k with off[k] <= v, over non-decreasing off[0..NS-1])
+template <int NS>
+__device__ __forceinline__ int nb_segment(const NbListT<NS>& L, int v)
 {
     int k = 0;
 #pragma unroll
-    for (int step = 16; step >= 1; step >>= 1) {
+    for (int step = NS > 64 ? 64 : 16; step >= 1; step >>= 1) {
         int t = k + step;
-        if (t < 27 && L.off[t] <= v) k = t;
+        if (t < NS && L.off[t] <= v) k = t;
     }
     return k;
 }
@@ -547,125 +588,140 @@ __device__ __forceinline__ void density_finish(const RunDev& R, int r, StateBuf&
 // ------------------------------------------------------------------------------------------------
 // With the Wendland kernel the pass fits 64 VGPRs without spilling: 8 waves per SIMD instead of 7,
 // 5% faster on the 2M-particle turbulence step. The other kernels keep the compiler's choice
-// (a cap spills them), as does the balanced pass with 256-candidate chunks, whose LDS allows 6.
-template <int CH, bool MIXED, bool MHD, int KT, bool BAL>
+// (a cap spills them), as do the balanced pass and the 125-cell stencil with 256-candidate chunks,
+// whose LDS allows 6.
+template <int CH, bool MIXED, bool MHD, int KT, bool BAL, int RMAX>
+__device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ runs, int nruns, int ncell_tot,
+                                             int nblk_total, StateBuf& S, WorkBuf& W, int kmax, NbListR<RMAX>& L,
+                                             float4 (&sp)[CH], ListT<CH> (&lst)[BAL ? 64 * 32 : 1])
+{
+    constexpr int NS = RMAX == 1 ? 27 : 125;
+    const int cell = xcd_cell(vb, nblk_total, ncell_tot);
+    if (cell < 0) return;
+    const int t0 = (int)W.cs[cell];
+    const int nt = (int)W.cs[cell + 1] - t0;
+    if (nt == 0) return;
+    const int r = find_run_by_cell(runs, nruns, cell);
+    const RunDev& R = runs[r];
+    build_nblist<MIXED, RMAX>(R, cell - R.c0, W.cs, L);
+    const int total = L.off[NS];
+    const int self_v0 = L.off[self_segment<RMAX>(R)];
+    const int lane = threadIdx.x;
+    const LaneMap lm = lane_map(nt, kmax);
+    const int K = lm.K, k = lm.k;
+    const int per = CH / K;   // lane candidates per chunk
+    const int chk = per * K;  // staged entries per chunk (a multiple of K)
+    const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
+    const float H2 = R.H2;
+    const float gam = R.gamma;
+    const bool keep = masks_fit(nt, total, lm);
+    MaskWriter mw;
+    mw.row = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
+    mw.pend = 0;
+    mw.pend_w = -1;
+
+    for (int tg = 0; tg < nt; tg += lm.tpg) {
+        const int t = tg + lm.tl;
+        const bool valid = lm.inwave && t < nt;
+        const int i = t0 + (valid ? t : 0);
+        const uint8_t live = S.mask[i], spc = S.species[i];
+        const bool tgt = valid && is_target(spc, live);
+        const float4 own = W.P0[i];
+        const float h = S.h[i];
+        // inputs of the epilogue, loaded early so their latency overlaps the cull
+        const float uu = S.u[i];
+        const float vx = S.vel[0][i], vy = S.vel[1][i], vz = S.vel[2][i];
+        const float rho_in = S.rho[i];
+        float bx = 0.f, by = 0.f, bz = 0.f, psi = 0.f;
+        if (MHD && R.mhd) {
+            bx = S.B[0][i];
+            by = S.B[1][i];
+            bz = S.B[2][i];
+            psi = S.psi[i];
+        }
+        const float invh = 1.0f / h;
+        const int self_v = self_v0 + t;
+        float acc = 0.0f;
+        uint32_t cnt = 0;
+
+        for (int base = 0; base < total; base += chk) {
+            for (int e = lane; e < chk; e += 64) {
+                int v = base + e;
+                float4 p = make_float4(kSentinel, kSentinel, kSentinel, 0.0f);
+                if (v < total) {
+                    int sg = nb_segment(L, v);
+                    p = W.P0[L.start[sg] + (v - L.off[sg])];
+                    p.x += L.sh[sg][0];
+                    p.y += L.sh[sg][1];
+                    p.z += L.sh[sg][2];
+                }
+                sp[e] = p;
+            }
+            __syncthreads();
+            const int cbase = base / K;
+            for (int sb = 0; sb < per; sb += 32) {
+                const int lim = per - sb < 32 ? per - sb : 32;
+                uint32_t msk = cull32(sp, k + sb * K, K, lim, own.x, own.y, own.z, H2);
+                // the self pair (r2 = 0) passes the cull: drop it here; its density term is the
+                // self term m_i W(0, h_i) added below, as DensityInitSelf does
+                const int sc = self_v - k;
+                if (sc >= 0 && sc % K == 0) {
+                    const int bit = sc / K - cbase - sb;
+                    if (bit >= 0 && bit < lim) msk &= ~(1u << bit);
+                }
+                if (!tgt) msk = 0;
+                if (keep) mw.put(msk, cbase + sb, lim);
+                cnt += (uint32_t)__popc(msk);
+                if (!BAL || K == 1) {
+                    while (msk) {
+                        const int ii = __builtin_ctz(msk);
+                        msk &= msk - 1;
+                        const float4 p = sp[k + (int)__umul24((unsigned)(sb + ii), (unsigned)K)];
+                        float dx, dy, dz;
+                        const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
+                        const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
+                        acc += p.w * kshape(kern, r2 * invR * invh);
+                    }
+                } else {
+                    const int gtot = compact_group(msk, sb, lm, lst);
+                    __syncthreads();
+                    for (int q = k; q < gtot; q += K) {
+                        const float4 p = sp[lst[lm.gfirst * 32 + q]];
+                        float dx, dy, dz;
+                        const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
+                        const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
+                        acc += p.w * kshape(kern, r2 * invR * invh);
+                    }
+                    __syncthreads();
+                }
+            }
+            __syncthreads();
+        }
+        if (keep) mw.flush();
+        density_finish<MHD>(R, r, S, W, lm, acc, cnt, valid, tgt, spc, live, i, own.w, h, invh, uu, vx, vy, vz,
+                            rho_in, bx, by, bz, psi, gam, kern);
+    }
+}
+
+template <int CH, bool MIXED, bool MHD, int KT, bool BAL, int RMAX = 1, bool GRID = false>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu((KT == 2 && !(BAL && CH > 128)) ? 8 : 1)))  // KT 2: Wendland
+__attribute__((amdgpu_waves_per_eu((KT == 2 && !((BAL || RMAX == 2) && CH > 128)) ? 8 : 1)))  // KT 2: Wendland
 void k_density(const RunDev* __restrict__ runs, int nruns, int ncell_tot, int nblk_total, StateBuf S, WorkBuf W,
                int kmax, int balance)
 {
-    __shared__ NbList L;
+    __shared__ NbListR<RMAX> L;
     __shared__ float4 sp[CH];
-    using LT = typename std::conditional<(CH <= 256), uint8_t, uint16_t>::type;
-    __shared__ LT lst[BAL ? 64 * 32 : 1];  // compaction lists of the balanced pair loop
-    // one supercell per wave, or a persistent grid walking cells at a stride of the grid
-    // (a multiple of 8, so every cell a wave visits belongs to its own XCD's range)
-    for (int vb = blockIdx.x; vb < nblk_total; vb += gridDim.x) {
-        const int cell = xcd_cell(vb, nblk_total, ncell_tot);
-        if (cell < 0) continue;
-        const int t0 = (int)W.cs[cell];
-        const int nt = (int)W.cs[cell + 1] - t0;
-        if (nt == 0) continue;
-        const int r = find_run_by_cell(runs, nruns, cell);
-        const RunDev& R = runs[r];
-        build_nblist<MIXED>(R, cell - R.c0, W.cs, L);
-        const int total = L.off[27];
-        const int self_v0 = L.off[13];
-        const int lane = threadIdx.x;
-        const LaneMap lm = lane_map(nt, kmax);
-        const int K = lm.K, k = lm.k;
-        const int per = CH / K;   // lane candidates per chunk
-        const int chk = per * K;  // staged entries per chunk (a multiple of K)
-        const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
-        const float H2 = R.H2;
-        const float gam = R.gamma;
-        const bool keep = masks_fit(nt, total, lm);
-        MaskWriter mw;
-        mw.row = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
-        mw.pend = 0;
-        mw.pend_w = -1;
-
-        for (int tg = 0; tg < nt; tg += lm.tpg) {
-            const int t = tg + lm.tl;
-            const bool valid = lm.inwave && t < nt;
-            const int i = t0 + (valid ? t : 0);
-            const uint8_t live = S.mask[i], spc = S.species[i];
-            const bool tgt = valid && is_target(spc, live);
-            const float4 own = W.P0[i];
-            const float h = S.h[i];
-            // inputs of the epilogue, loaded early so their latency overlaps the cull
-            const float uu = S.u[i];
-            const float vx = S.vel[0][i], vy = S.vel[1][i], vz = S.vel[2][i];
-            const float rho_in = S.rho[i];
-            float bx = 0.f, by = 0.f, bz = 0.f, psi = 0.f;
-            if (MHD && R.mhd) {
-                bx = S.B[0][i];
-                by = S.B[1][i];
-                bz = S.B[2][i];
-                psi = S.psi[i];
-            }
-            const float invh = 1.0f / h;
-            const int self_v = self_v0 + t;
-            float acc = 0.0f;
-            uint32_t cnt = 0;
-
-            for (int base = 0; base < total; base += chk) {
-                for (int e = lane; e < chk; e += 64) {
-                    int v = base + e;
-                    float4 p = make_float4(kSentinel, kSentinel, kSentinel, 0.0f);
-                    if (v < total) {
-                        int sg = nb_segment(L, v);
-                        p = W.P0[L.start[sg] + (v - L.off[sg])];
-                        p.x += L.sh[sg][0];
-                        p.y += L.sh[sg][1];
-                        p.z += L.sh[sg][2];
-                    }
-                    sp[e] = p;
-                }
-                __syncthreads();
-                const int cbase = base / K;
-                for (int sb = 0; sb < per; sb += 32) {
-                    const int lim = per - sb < 32 ? per - sb : 32;
-                    uint32_t msk = cull32(sp, k + sb * K, K, lim, own.x, own.y, own.z, H2);
-                    // the self pair (r2 = 0) passes the cull: drop it here; its density term is the
-                    // self term m_i W(0, h_i) added below, as DensityInitSelf does
-                    const int sc = self_v - k;
-                    if (sc >= 0 && sc % K == 0) {
-                        const int bit = sc / K - cbase - sb;
-                        if (bit >= 0 && bit < lim) msk &= ~(1u << bit);
-                    }
-                    if (!tgt) msk = 0;
-                    if (keep) mw.put(msk, cbase + sb, lim);
-                    cnt += (uint32_t)__popc(msk);
-                    if (!BAL || K == 1) {
-                        while (msk) {
-                            const int ii = __builtin_ctz(msk);
-                            msk &= msk - 1;
-                            const float4 p = sp[k + (int)__umul24((unsigned)(sb + ii), (unsigned)K)];
-                            float dx, dy, dz;
-                            const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
-                            const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
-                            acc += p.w * kshape(kern, r2 * invR * invh);
-                        }
-                    } else {
-                        const int gtot = compact_group(msk, sb, lm, lst);
-                        __syncthreads();
-                        for (int q = k; q < gtot; q += K) {
-                            const float4 p = sp[lst[lm.gfirst * 32 + q]];
-                            float dx, dy, dz;
-                            const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
-                            const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
-                            acc += p.w * kshape(kern, r2 * invR * invh);
-                        }
-                        __syncthreads();
-                    }
-                }
-                __syncthreads();
-            }
-            if (keep) mw.flush();
-            density_finish<MHD>(R, r, S, W, lm, acc, cnt, valid, tgt, spc, live, i, own.w, h, invh, uu, vx, vy, vz,
-                                rho_in, bx, by, bz, psi, gam, kern);
-        }
+    __shared__ ListT<CH> lst[BAL ? 64 * 32 : 1];  // compaction lists of the balanced pair loop
+    if constexpr (GRID) {
+        // a persistent grid: each wave walks the supercells at a stride of the grid (a multiple
+        // of 8, so every cell a wave visits belongs to its own XCD's range)
+        for (int vb = blockIdx.x; vb < nblk_total; vb += gridDim.x)
+            density_wave<CH, MIXED, MHD, KT, BAL, RMAX>(vb, runs, nruns, ncell_tot, nblk_total, S, W, kmax, L, sp,
+                                                        lst);
+    } else {
+        // one supercell per wave
+        density_wave<CH, MIXED, MHD, KT, BAL, RMAX>((int)blockIdx.x, runs, nruns, ncell_tot, nblk_total, S, W, kmax,
+                                                    L, sp, lst);
     }
 }
 
@@ -831,119 +887,133 @@ __device__ __forceinline__ void force_finish(const RunDev& R, WorkBuf& W, const 
     }
 }
 
-template <int CH, bool MIXED, bool MHD, int KT, int BAL>
+template <int CH, bool MIXED, bool MHD, int KT, int BAL, int RMAX>
+__device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ runs, int nruns, int ncell_tot,
+                                            int nblk_total, StateBuf& S, WorkBuf& W, int kmax, NbListR<RMAX>& L,
+                                            float4 (&sp)[MHD ? 5 : 4][CH], ListT<CH> (&lst)[BAL == 1 ? 64 * 32 : 1])
+{
+    constexpr int NS = RMAX == 1 ? 27 : 125;
+    const int cell = xcd_cell(vb, nblk_total, ncell_tot);
+    if (cell < 0) return;
+    const int t0 = (int)W.cs[cell];
+    const int nt = (int)W.cs[cell + 1] - t0;
+    if (nt == 0) return;
+    const int r = find_run_by_cell(runs, nruns, cell);
+    const RunDev& R = runs[r];
+    build_nblist<MIXED, RMAX>(R, cell - R.c0, W.cs, L);
+    const int total = L.off[NS];
+    const int lane = threadIdx.x;
+    const LaneMap lm = lane_map(nt, kmax);
+    const int K = lm.K, k = lm.k;
+    const int per = CH / K;
+    const int chk = per * K;
+    const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
+    const float H2 = R.H2;
+    const float alpha = R.alpha, beta = R.beta;
+    const bool mhd = MHD && R.mhd;
+    const bool clean = mhd && R.cleaning;
+    const float borve = mhd ? R.borve : 0.0f;
+    const float ch = __uint_as_float(W.chbits[r]);
+    const bool keep = masks_fit(nt, total, lm);
+    const int per_lane = (total + K - 1) / K;
+    const uint32_t* mrow = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
+
+    for (int tg = 0; tg < nt; tg += lm.tpg) {
+        const int t = tg + lm.tl;
+        const bool valid = lm.inwave && t < nt;
+        const int i = t0 + (valid ? t : 0);
+        const bool tgt = valid && is_target(S.species[i], S.mask[i]);
+        const float4 o0 = W.P0[i], o1 = W.P1[i], o2 = W.P2[i], o3 = W.P3[i];
+        float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (MHD) o4 = W.P4[i];
+        const int self_v = L.off[self_segment<RMAX>(R)] + t;
+        ForceAcc A = {};
+
+        for (int base = 0; base < total; base += chk) {
+            for (int e = lane; e < chk; e += 64) {
+                int v = base + e;
+                if (v < total) {
+                    int sg = nb_segment(L, v);
+                    int j = L.start[sg] + (v - L.off[sg]);
+                    float4 p = W.P0[j];
+                    p.x += L.sh[sg][0];
+                    p.y += L.sh[sg][1];
+                    p.z += L.sh[sg][2];
+                    sp[0][e] = p;
+                    sp[1][e] = W.P1[j];
+                    sp[2][e] = W.P2[j];
+                    sp[3][e] = W.P3[j];
+                    if constexpr (MHD) sp[4][e] = W.P4[j];
+                } else {
+                    sp[0][e] = make_float4(kSentinel, kSentinel, kSentinel, 0.0f);
+                }
+            }
+            __syncthreads();
+            const int cbase = base / K;
+            for (int sb = 0; sb < per; sb += 32) {
+                const int lim = per - sb < 32 ? per - sb : 32;
+                uint32_t msk;
+                if (keep) {
+                    msk = mask_get(mrow, cbase + sb, lim, per_lane);
+                } else {
+                    msk = cull32(sp[0], k + sb * K, K, lim, o0.x, o0.y, o0.z, H2);
+                    const int sc = self_v - k;  // the self pair adds exact zeros; drop it anyway
+                    if (sc >= 0 && sc % K == 0) {
+                        const int bit = sc / K - cbase - sb;
+                        if (bit >= 0 && bit < lim) msk &= ~(1u << bit);
+                    }
+                }
+                if (!tgt) msk = 0;
+                if (BAL == 0 || K == 1) {
+                    while (msk) {
+                        const int ii = __builtin_ctz(msk);
+                        msk &= msk - 1;
+                        const int e = k + (int)__umul24((unsigned)(sb + ii), (unsigned)K);
+                        float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if constexpr (MHD) q4 = sp[4][e];
+                        // all of the pair's records read before any is used: one LDS wait per pair
+                        // instead of one per record (force pass 3% faster; same arithmetic)
+                        const float4 q0 = sp[0][e], q1 = sp[1][e], q2 = sp[2][e], q3 = sp[3][e];
+                        __builtin_amdgcn_sched_barrier(0);
+                        force_pair<MHD>(o0, o1, o2, o3, o4, q0, q1, q2, q3, q4, kern, alpha, beta, mhd, clean, A);
+                    }
+                } else {
+                    const int gtot = compact_group(msk, sb, lm, lst);
+                    __syncthreads();
+                    for (int q = k; q < gtot; q += K) {
+                        const int e = lst[lm.gfirst * 32 + q];
+                        float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if constexpr (MHD) q4 = sp[4][e];
+                        force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
+                                        beta, mhd, clean, A);
+                    }
+                    __syncthreads();
+                }
+            }
+            __syncthreads();
+        }
+        force_finish<MHD>(R, W, lm, A, tgt, i, o1, o2, o3, o4, mhd, clean, borve, ch);
+    }
+}
+
+template <int CH, bool MIXED, bool MHD, int KT, int BAL, int RMAX = 1, bool GRID = false>
 __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
                                                int nblk_total, StateBuf S, WorkBuf W, int kmax, int balance)
 {
     constexpr int NP = MHD ? 5 : 4;
-    __shared__ NbList L;
+    __shared__ NbListR<RMAX> L;
     __shared__ float4 sp[NP][CH];
-    using LT = typename std::conditional<(CH <= 256), uint8_t, uint16_t>::type;
-    __shared__ LT lst[BAL == 1 ? 64 * 32 : 1];  // compaction lists of the balanced pair loop
-    // one supercell per wave, or a persistent grid walking cells at a stride of the grid
-    // (a multiple of 8, so every cell a wave visits belongs to its own XCD's range)
-    for (int vb = blockIdx.x; vb < nblk_total; vb += gridDim.x) {
-        const int cell = xcd_cell(vb, nblk_total, ncell_tot);
-        if (cell < 0) continue;
-        const int t0 = (int)W.cs[cell];
-        const int nt = (int)W.cs[cell + 1] - t0;
-        if (nt == 0) continue;
-        const int r = find_run_by_cell(runs, nruns, cell);
-        const RunDev& R = runs[r];
-        build_nblist<MIXED>(R, cell - R.c0, W.cs, L);
-        const int total = L.off[27];
-        const int lane = threadIdx.x;
-        const LaneMap lm = lane_map(nt, kmax);
-        const int K = lm.K, k = lm.k;
-        const int per = CH / K;
-        const int chk = per * K;
-        const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
-        const float H2 = R.H2;
-        const float alpha = R.alpha, beta = R.beta;
-        const bool mhd = MHD && R.mhd;
-        const bool clean = mhd && R.cleaning;
-        const float borve = mhd ? R.borve : 0.0f;
-        const float ch = __uint_as_float(W.chbits[r]);
-        const bool keep = masks_fit(nt, total, lm);
-        const int per_lane = (total + K - 1) / K;
-        const uint32_t* mrow = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
-
-        for (int tg = 0; tg < nt; tg += lm.tpg) {
-            const int t = tg + lm.tl;
-            const bool valid = lm.inwave && t < nt;
-            const int i = t0 + (valid ? t : 0);
-            const bool tgt = valid && is_target(S.species[i], S.mask[i]);
-            const float4 o0 = W.P0[i], o1 = W.P1[i], o2 = W.P2[i], o3 = W.P3[i];
-            float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (MHD) o4 = W.P4[i];
-            const int self_v = L.off[13] + t;
-            ForceAcc A = {};
-
-            for (int base = 0; base < total; base += chk) {
-                for (int e = lane; e < chk; e += 64) {
-                    int v = base + e;
-                    if (v < total) {
-                        int sg = nb_segment(L, v);
-                        int j = L.start[sg] + (v - L.off[sg]);
-                        float4 p = W.P0[j];
-                        p.x += L.sh[sg][0];
-                        p.y += L.sh[sg][1];
-                        p.z += L.sh[sg][2];
-                        sp[0][e] = p;
-                        sp[1][e] = W.P1[j];
-                        sp[2][e] = W.P2[j];
-                        sp[3][e] = W.P3[j];
-                        if constexpr (MHD) sp[4][e] = W.P4[j];
-                    } else {
-                        sp[0][e] = make_float4(kSentinel, kSentinel, kSentinel, 0.0f);
-                    }
-                }
-                __syncthreads();
-                const int cbase = base / K;
-                for (int sb = 0; sb < per; sb += 32) {
-                    const int lim = per - sb < 32 ? per - sb : 32;
-                    uint32_t msk;
-                    if (keep) {
-                        msk = mask_get(mrow, cbase + sb, lim, per_lane);
-                    } else {
-                        msk = cull32(sp[0], k + sb * K, K, lim, o0.x, o0.y, o0.z, H2);
-                        const int sc = self_v - k;  // the self pair adds exact zeros; drop it anyway
-                        if (sc >= 0 && sc % K == 0) {
-                            const int bit = sc / K - cbase - sb;
-                            if (bit >= 0 && bit < lim) msk &= ~(1u << bit);
-                        }
-                    }
-                    if (!tgt) msk = 0;
-                    if (BAL == 0 || K == 1) {
-                        while (msk) {
-                            const int ii = __builtin_ctz(msk);
-                            msk &= msk - 1;
-                            const int e = k + (int)__umul24((unsigned)(sb + ii), (unsigned)K);
-                            float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                            if constexpr (MHD) q4 = sp[4][e];
-                            // all of the pair's records read before any is used: one LDS wait per pair
-                            // instead of one per record (force pass 3% faster; same arithmetic)
-                            const float4 q0 = sp[0][e], q1 = sp[1][e], q2 = sp[2][e], q3 = sp[3][e];
-                            __builtin_amdgcn_sched_barrier(0);
-                            force_pair<MHD>(o0, o1, o2, o3, o4, q0, q1, q2, q3, q4, kern, alpha, beta, mhd, clean, A);
-                        }
-                    } else {
-                        const int gtot = compact_group(msk, sb, lm, lst);
-                        __syncthreads();
-                        for (int q = k; q < gtot; q += K) {
-                            const int e = lst[lm.gfirst * 32 + q];
-                            float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                            if constexpr (MHD) q4 = sp[4][e];
-                            force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
-                                            beta, mhd, clean, A);
-                        }
-                        __syncthreads();
-                    }
-                }
-                __syncthreads();
-            }
-            force_finish<MHD>(R, W, lm, A, tgt, i, o1, o2, o3, o4, mhd, clean, borve, ch);
-        }
+    __shared__ ListT<CH> lst[BAL == 1 ? 64 * 32 : 1];  // compaction lists of the balanced pair loop
+    if constexpr (GRID) {
+        // a persistent grid: each wave walks the supercells at a stride of the grid (a multiple
+        // of 8, so every cell a wave visits belongs to its own XCD's range)
+        for (int vb = blockIdx.x; vb < nblk_total; vb += gridDim.x)
+            forces_wave<CH, MIXED, MHD, KT, BAL, RMAX>(vb, runs, nruns, ncell_tot, nblk_total, S, W, kmax, L, sp,
+                                                       lst);
+    } else {
+        // one supercell per wave
+        forces_wave<CH, MIXED, MHD, KT, BAL, RMAX>((int)blockIdx.x, runs, nruns, ncell_tot, nblk_total, S, W, kmax,
+                                                   L, sp, lst);
     }
 }
 
@@ -990,94 +1060,91 @@ template <int CAPD, bool MIXED, bool MHD, int KT>
 __global__ __launch_bounds__(64) void k_density_cell(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
                                                      int nblk_total, StateBuf S, WorkBuf W, int kmax)
 {
-    __shared__ NbList L;
+    constexpr int RMAX = 1, NS = 27;
+    __shared__ NbListR<RMAX> L;
     __shared__ float4 sp[CAPD];
-    // one supercell per wave, or a persistent grid walking cells at a stride of the grid
-    // (a multiple of 8, so every cell a wave visits belongs to its own XCD's range)
-    for (int vb = blockIdx.x; vb < nblk_total; vb += gridDim.x) {
-        const int cell = xcd_cell(vb, nblk_total, ncell_tot);
-        if (cell < 0) continue;
-        const int t0 = (int)W.cs[cell];
-        const int nt = (int)W.cs[cell + 1] - t0;
-        if (nt == 0) continue;
-        const int r = find_run_by_cell(runs, nruns, cell);
-        const RunDev& R = runs[r];
-        build_nblist<MIXED>(R, cell - R.c0, W.cs, L);
-        const int total = L.off[27];
-        const int lane = threadIdx.x;
-        const LaneMap lm = lane_map(nt, kmax);
-        const int K = lm.K, k = lm.k;
-        const int Lr = (total + K - 1) / K;
-        const bool keep = nt <= lm.tpg && Lr <= 32 * kMaskWords;
-        const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
-        const float H2 = R.H2;
-        const float gam = R.gamma;
-        MaskWriter mw;
-        mw.row = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
-        mw.pend = 0;
-        mw.pend_w = -1;
+    const int cell = xcd_cell(blockIdx.x, nblk_total, ncell_tot);
+    if (cell < 0) return;
+    const int t0 = (int)W.cs[cell];
+    const int nt = (int)W.cs[cell + 1] - t0;
+    if (nt == 0) return;
+    const int r = find_run_by_cell(runs, nruns, cell);
+    const RunDev& R = runs[r];
+    build_nblist<MIXED, RMAX>(R, cell - R.c0, W.cs, L);
+    const int total = L.off[NS];
+    const int lane = threadIdx.x;
+    const LaneMap lm = lane_map(nt, kmax);
+    const int K = lm.K, k = lm.k;
+    const int Lr = (total + K - 1) / K;
+    const bool keep = nt <= lm.tpg && Lr <= 32 * kMaskWords;
+    const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
+    const float H2 = R.H2;
+    const float gam = R.gamma;
+    MaskWriter mw;
+    mw.row = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
+    mw.pend = 0;
+    mw.pend_w = -1;
 
-        for (int tg = 0; tg < nt; tg += lm.tpg) {
-            const int t = tg + lm.tl;
-            const bool valid = lm.inwave && t < nt;
-            const int i = t0 + (valid ? t : 0);
-            const uint8_t live = S.mask[i], spc = S.species[i];
-            const bool tgt = valid && is_target(spc, live);
-            const float4 own = W.P0[i];
-            const float h = S.h[i];
-            const float uu = S.u[i];
-            const float vx = S.vel[0][i], vy = S.vel[1][i], vz = S.vel[2][i];
-            const float rho_in = S.rho[i];
-            float bx = 0.f, by = 0.f, bz = 0.f, psi = 0.f;
-            if (MHD && R.mhd) {
-                bx = S.B[0][i];
-                by = S.B[1][i];
-                bz = S.B[2][i];
-                psi = S.psi[i];
-            }
-            const float invh = 1.0f / h;
-            const int self_v = L.off[13] + t;
-            const int v0 = lm.inwave ? k * Lr : total;
-            const int v1 = v0 + Lr < total ? v0 + Lr : total;
-            float acc = 0.0f;
-            uint32_t cnt = 0;
-            for (int base = 0; base < total; base += CAPD) {
-                const int top = base + CAPD < total ? base + CAPD : total;
-                for (int v = base + lane; v < top; v += 64) {
-                    const int sg = nb_segment(L, v);
-                    float4 p = W.P0[L.start[sg] + (v - L.off[sg])];
-                    p.x += L.sh[sg][0];
-                    p.y += L.sh[sg][1];
-                    p.z += L.sh[sg][2];
-                    sp[v - base] = p;
-                }
-                __syncthreads();
-                const int a = v0 > base ? v0 : base;
-                const int b = v1 < top ? v1 : top;
-                for (int cc = a; cc < b; cc += 32) {
-                    const int lim = b - cc < 32 ? b - cc : 32;
-                    uint32_t msk = cull32(sp, cc - base, 1, lim, own.x, own.y, own.z, H2);
-                    // the self pair passes the cull; its density term is the self term added at the end
-                    if (self_v >= cc && self_v < cc + lim) msk &= ~(1u << (self_v - cc));
-                    if (!tgt) msk = 0;
-                    if (keep) mw.put(msk, cc - v0, lim);
-                    cnt += (uint32_t)__popc(msk);
-                    while (msk) {
-                        const int ii = __builtin_ctz(msk);
-                        msk &= msk - 1;
-                        const float4 p = sp[cc - base + ii];
-                        float dx, dy, dz;
-                        const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
-                        const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
-                        acc += p.w * kshape(kern, r2 * invR * invh);
-                    }
-                }
-                __syncthreads();
-            }
-            if (keep) mw.flush();
-            density_finish<MHD>(R, r, S, W, lm, acc, cnt, valid, tgt, spc, live, i, own.w, h, invh, uu, vx, vy, vz,
-                                rho_in, bx, by, bz, psi, gam, kern);
+    for (int tg = 0; tg < nt; tg += lm.tpg) {
+        const int t = tg + lm.tl;
+        const bool valid = lm.inwave && t < nt;
+        const int i = t0 + (valid ? t : 0);
+        const uint8_t live = S.mask[i], spc = S.species[i];
+        const bool tgt = valid && is_target(spc, live);
+        const float4 own = W.P0[i];
+        const float h = S.h[i];
+        const float uu = S.u[i];
+        const float vx = S.vel[0][i], vy = S.vel[1][i], vz = S.vel[2][i];
+        const float rho_in = S.rho[i];
+        float bx = 0.f, by = 0.f, bz = 0.f, psi = 0.f;
+        if (MHD && R.mhd) {
+            bx = S.B[0][i];
+            by = S.B[1][i];
+            bz = S.B[2][i];
+            psi = S.psi[i];
         }
+        const float invh = 1.0f / h;
+        const int self_v = L.off[self_segment<RMAX>(R)] + t;
+        const int v0 = lm.inwave ? k * Lr : total;
+        const int v1 = v0 + Lr < total ? v0 + Lr : total;
+        float acc = 0.0f;
+        uint32_t cnt = 0;
+        for (int base = 0; base < total; base += CAPD) {
+            const int top = base + CAPD < total ? base + CAPD : total;
+            for (int v = base + lane; v < top; v += 64) {
+                const int sg = nb_segment(L, v);
+                float4 p = W.P0[L.start[sg] + (v - L.off[sg])];
+                p.x += L.sh[sg][0];
+                p.y += L.sh[sg][1];
+                p.z += L.sh[sg][2];
+                sp[v - base] = p;
+            }
+            __syncthreads();
+            const int a = v0 > base ? v0 : base;
+            const int b = v1 < top ? v1 : top;
+            for (int cc = a; cc < b; cc += 32) {
+                const int lim = b - cc < 32 ? b - cc : 32;
+                uint32_t msk = cull32(sp, cc - base, 1, lim, own.x, own.y, own.z, H2);
+                // the self pair passes the cull; its density term is the self term added at the end
+                if (self_v >= cc && self_v < cc + lim) msk &= ~(1u << (self_v - cc));
+                if (!tgt) msk = 0;
+                if (keep) mw.put(msk, cc - v0, lim);
+                cnt += (uint32_t)__popc(msk);
+                while (msk) {
+                    const int ii = __builtin_ctz(msk);
+                    msk &= msk - 1;
+                    const float4 p = sp[cc - base + ii];
+                    float dx, dy, dz;
+                    const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
+                    const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
+                    acc += p.w * kshape(kern, r2 * invR * invh);
+                }
+            }
+            __syncthreads();
+        }
+        if (keep) mw.flush();
+        density_finish<MHD>(R, r, S, W, lm, acc, cnt, valid, tgt, spc, live, i, own.w, h, invh, uu, vx, vy, vz,
+                            rho_in, bx, by, bz, psi, gam, kern);
     }
 }
 
@@ -1086,132 +1153,129 @@ __global__ __launch_bounds__(64) void k_forces_cell(const RunDev* __restrict__ r
                                                     int nblk_total, StateBuf S, WorkBuf W, int kmax)
 {
     constexpr int NP = MHD ? 5 : 4;
-    __shared__ NbList L;
+    constexpr int RMAX = 1, NS = 27;
+    __shared__ NbListR<RMAX> L;
     __shared__ float4 sp[NP][CH];
     __shared__ uint16_t lst[kCellListCap];
-    // one supercell per wave, or a persistent grid walking cells at a stride of the grid
-    // (a multiple of 8, so every cell a wave visits belongs to its own XCD's range)
-    for (int vb = blockIdx.x; vb < nblk_total; vb += gridDim.x) {
-        const int cell = xcd_cell(vb, nblk_total, ncell_tot);
-        if (cell < 0) continue;
-        const int t0 = (int)W.cs[cell];
-        const int nt = (int)W.cs[cell + 1] - t0;
-        if (nt == 0) continue;
-        const int r = find_run_by_cell(runs, nruns, cell);
-        const RunDev& R = runs[r];
-        build_nblist<MIXED>(R, cell - R.c0, W.cs, L);
-        const int total = L.off[27];
-        const int lane = threadIdx.x;
-        const LaneMap lm = lane_map(nt, kmax);
-        const int K = lm.K, k = lm.k;
-        const int Lr = (total + K - 1) / K;
-        const bool keep = nt <= lm.tpg && Lr <= 32 * kMaskWords;  // the density pass stored masks
-        const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
-        const float H2 = R.H2;
-        const float alpha = R.alpha, beta = R.beta;
-        const bool mhd = MHD && R.mhd;
-        const bool clean = mhd && R.cleaning;
-        const float borve = mhd ? R.borve : 0.0f;
-        const float ch = __uint_as_float(W.chbits[r]);
-        const uint32_t* mrow = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
+    const int cell = xcd_cell(blockIdx.x, nblk_total, ncell_tot);
+    if (cell < 0) return;
+    const int t0 = (int)W.cs[cell];
+    const int nt = (int)W.cs[cell + 1] - t0;
+    if (nt == 0) return;
+    const int r = find_run_by_cell(runs, nruns, cell);
+    const RunDev& R = runs[r];
+    build_nblist<MIXED, RMAX>(R, cell - R.c0, W.cs, L);
+    const int total = L.off[NS];
+    const int lane = threadIdx.x;
+    const LaneMap lm = lane_map(nt, kmax);
+    const int K = lm.K, k = lm.k;
+    const int Lr = (total + K - 1) / K;
+    const bool keep = nt <= lm.tpg && Lr <= 32 * kMaskWords;  // the density pass stored masks
+    const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
+    const float H2 = R.H2;
+    const float alpha = R.alpha, beta = R.beta;
+    const bool mhd = MHD && R.mhd;
+    const bool clean = mhd && R.cleaning;
+    const float borve = mhd ? R.borve : 0.0f;
+    const float ch = __uint_as_float(W.chbits[r]);
+    const uint32_t* mrow = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
 
-        for (int tg = 0; tg < nt; tg += lm.tpg) {
-            const int t = tg + lm.tl;
-            const bool valid = lm.inwave && t < nt;
-            const int i = t0 + (valid ? t : 0);
-            const bool tgt = valid && is_target(S.species[i], S.mask[i]);
-            const float4 o0 = W.P0[i], o1 = W.P1[i], o2 = W.P2[i], o3 = W.P3[i];
-            float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (MHD) o4 = W.P4[i];
-            const int self_v = L.off[13] + t;
-            const int v0 = lm.inwave ? k * Lr : total;
-            const int v1 = v0 + Lr < total ? v0 + Lr : total;
-            ForceAcc A = {};
+    for (int tg = 0; tg < nt; tg += lm.tpg) {
+        const int t = tg + lm.tl;
+        const bool valid = lm.inwave && t < nt;
+        const int i = t0 + (valid ? t : 0);
+        const bool tgt = valid && is_target(S.species[i], S.mask[i]);
+        const float4 o0 = W.P0[i], o1 = W.P1[i], o2 = W.P2[i], o3 = W.P3[i];
+        float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (MHD) o4 = W.P4[i];
+        const int self_v = L.off[self_segment<RMAX>(R)] + t;
+        const int v0 = lm.inwave ? k * Lr : total;
+        const int v1 = v0 + Lr < total ? v0 + Lr : total;
+        ForceAcc A = {};
 
-            // neighbour lists from the stored masks: lane order = candidate order within a target
-            const int mine = (keep && tgt) ? mask_count(mrow, 0, v1 - v0) : 0;
-            const int incl = wave_incl_scan(mine);
-            const bool listed = keep && __shfl(incl, 63, 64) <= kCellListCap;
-            int lo = 0;
-            if (listed) {
-                int pos = incl - mine;
-                for (int w = 0; w * 32 < v1 - v0 && mine > 0; ++w) {
-                    const int lim = (v1 - v0) - w * 32 < 32 ? (v1 - v0) - w * 32 : 32;
-                    uint32_t m = mask_bits(mrow, w * 32, lim);
-                    while (m) {
-                        const int ii = __builtin_ctz(m);
-                        m &= m - 1;
-                        lst[pos++] = (uint16_t)(v0 + w * 32 + ii);
-                    }
+        // neighbour lists from the stored masks: lane order = candidate order within a target
+        const int mine = (keep && tgt) ? mask_count(mrow, 0, v1 - v0) : 0;
+        const int incl = wave_incl_scan(mine);
+        const bool listed = keep && __shfl(incl, 63, 64) <= kCellListCap;
+        int lo = 0;
+        if (listed) {
+            int pos = incl - mine;
+            for (int w = 0; w * 32 < v1 - v0 && mine > 0; ++w) {
+                const int lim = (v1 - v0) - w * 32 < 32 ? (v1 - v0) - w * 32 : 32;
+                uint32_t m = mask_bits(mrow, w * 32, lim);
+                while (m) {
+                    const int ii = __builtin_ctz(m);
+                    m &= m - 1;
+                    lst[pos++] = (uint16_t)(v0 + w * 32 + ii);
                 }
-                // every lane takes part in the shuffle (a lane reading from an inactive lane does not
-                // get that lane's value): clamp the source, then select
-                const int pre = __shfl(incl, lm.gfirst > 0 ? lm.gfirst - 1 : 0, 64);
-                lo = lm.gfirst > 0 ? pre : 0;  // first entry of this target's list
             }
+            // every lane takes part in the shuffle (a lane reading from an inactive lane does not
+            // get that lane's value): clamp the source, then select
+            const int pre = __shfl(incl, lm.gfirst > 0 ? lm.gfirst - 1 : 0, 64);
+            lo = lm.gfirst > 0 ? pre : 0;  // first entry of this target's list
+        }
 
-            for (int base = 0; base < total; base += CH) {
-                const int top = base + CH < total ? base + CH : total;
-                for (int v = base + lane; v < top; v += 64) {
-                    const int e = v - base;
-                    const int sg = nb_segment(L, v);
-                    const int j = L.start[sg] + (v - L.off[sg]);
-                    float4 p = W.P0[j];
-                    p.x += L.sh[sg][0];
-                    p.y += L.sh[sg][1];
-                    p.z += L.sh[sg][2];
-                    sp[0][e] = p;
-                    sp[1][e] = W.P1[j];
-                    sp[2][e] = W.P2[j];
-                    sp[3][e] = W.P3[j];
-                    if constexpr (MHD) sp[4][e] = W.P4[j];
+        for (int base = 0; base < total; base += CH) {
+            const int top = base + CH < total ? base + CH : total;
+            for (int v = base + lane; v < top; v += 64) {
+                const int e = v - base;
+                const int sg = nb_segment(L, v);
+                const int j = L.start[sg] + (v - L.off[sg]);
+                float4 p = W.P0[j];
+                p.x += L.sh[sg][0];
+                p.y += L.sh[sg][1];
+                p.z += L.sh[sg][2];
+                sp[0][e] = p;
+                sp[1][e] = W.P1[j];
+                sp[2][e] = W.P2[j];
+                sp[3][e] = W.P3[j];
+                if constexpr (MHD) sp[4][e] = W.P4[j];
+            }
+            __syncthreads();
+            if (listed) {
+                // this target's entries below `top`: counted from each sub-lane's own bits
+                const int a = v0 > base ? v0 : base;
+                const int b = v1 < top ? v1 : top;
+                const int in_chunk = (tgt && b > a) ? mask_count(mrow, a - v0, b - v0) : 0;
+                const int ci = wave_incl_scan(in_chunk);
+                const int cpre = __shfl(ci, lm.gfirst > 0 ? lm.gfirst - 1 : 0, 64);
+                const int clast = __shfl(ci, lm.glast, 64);
+                const int g = lm.inwave ? clast - (lm.gfirst > 0 ? cpre : 0) : 0;
+                for (int q = k; q < g; q += K) {
+                    const int e = (int)lst[lo + q] - base;
+                    float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if constexpr (MHD) q4 = sp[4][e];
+                    force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
+                                    beta, mhd, clean, A);
                 }
-                __syncthreads();
-                if (listed) {
-                    // this target's entries below `top`: counted from each sub-lane's own bits
-                    const int a = v0 > base ? v0 : base;
-                    const int b = v1 < top ? v1 : top;
-                    const int in_chunk = (tgt && b > a) ? mask_count(mrow, a - v0, b - v0) : 0;
-                    const int ci = wave_incl_scan(in_chunk);
-                    const int cpre = __shfl(ci, lm.gfirst > 0 ? lm.gfirst - 1 : 0, 64);
-                    const int clast = __shfl(ci, lm.glast, 64);
-                    const int g = lm.inwave ? clast - (lm.gfirst > 0 ? cpre : 0) : 0;
-                    for (int q = k; q < g; q += K) {
-                        const int e = (int)lst[lo + q] - base;
+                lo += g;
+            } else {
+                const int a = v0 > base ? v0 : base;
+                const int b = v1 < top ? v1 : top;
+                for (int cc = a; cc < b; cc += 32) {
+                    const int lim = b - cc < 32 ? b - cc : 32;
+                    uint32_t msk;
+                    if (keep) {
+                        msk = mask_bits(mrow, cc - v0, lim);
+                    } else {
+                        msk = cull32(sp[0], cc - base, 1, lim, o0.x, o0.y, o0.z, H2);
+                        if (self_v >= cc && self_v < cc + lim) msk &= ~(1u << (self_v - cc));
+                    }
+                    if (!tgt) msk = 0;
+                    while (msk) {
+                        const int ii = __builtin_ctz(msk);
+                        msk &= msk - 1;
+                        const int e = cc - base + ii;
                         float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
                         if constexpr (MHD) q4 = sp[4][e];
                         force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
                                         beta, mhd, clean, A);
                     }
-                    lo += g;
-                } else {
-                    const int a = v0 > base ? v0 : base;
-                    const int b = v1 < top ? v1 : top;
-                    for (int cc = a; cc < b; cc += 32) {
-                        const int lim = b - cc < 32 ? b - cc : 32;
-                        uint32_t msk;
-                        if (keep) {
-                            msk = mask_bits(mrow, cc - v0, lim);
-                        } else {
-                            msk = cull32(sp[0], cc - base, 1, lim, o0.x, o0.y, o0.z, H2);
-                            if (self_v >= cc && self_v < cc + lim) msk &= ~(1u << (self_v - cc));
-                        }
-                        if (!tgt) msk = 0;
-                        while (msk) {
-                            const int ii = __builtin_ctz(msk);
-                            msk &= msk - 1;
-                            const int e = cc - base + ii;
-                            float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                            if constexpr (MHD) q4 = sp[4][e];
-                            force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
-                                            beta, mhd, clean, A);
-                        }
-                    }
                 }
-                __syncthreads();
             }
-            force_finish<MHD>(R, W, lm, A, tgt, i, o1, o2, o3, o4, mhd, clean, borve, ch);
+            __syncthreads();
         }
+        force_finish<MHD>(R, W, lm, A, tgt, i, o1, o2, o3, o4, mhd, clean, borve, ch);
     }
 }
 
@@ -1410,6 +1474,55 @@ __global__ __launch_bounds__(256) void k_move_runs(StateBuf A, StateBuf B, const
     if (mhd) B.psi[d] = A.psi[s];
 }
 
+// Per-run scalars across a change of the run set: the old runs' dt, time and priming flag are saved,
+// then new run r takes those of old run run_src[r], or starts fresh when run_src[r] < 0 (dt = its
+// fixed dt or 0 until its priming step sets one, time 0, primed by its next step).
+__global__ __launch_bounds__(256) void k_save_run_scalars(WorkBuf W, int n_old)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= n_old) return;
+    W.run_tmp[3 * q] = W.dt[q];
+    W.run_tmp[3 * q + 1] = W.time[q];
+    W.run_tmp[3 * q + 2] = (double)W.prime_flag[q];
+}
+
+__global__ __launch_bounds__(256) void k_load_run_scalars(const RunDev* __restrict__ runs, WorkBuf W, int n_new)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n_new) return;
+    const int q = W.run_src[r];
+    if (q >= 0) {
+        W.dt[r] = W.run_tmp[3 * q];
+        W.time[r] = W.run_tmp[3 * q + 1];
+        W.prime_flag[r] = (int32_t)W.run_tmp[3 * q + 2];
+    } else {
+        W.dt[r] = runs[r].dt_fixed > 0.0 ? runs[r].dt_fixed : 0.0;
+        W.time[r] = 0.0;
+        W.prime_flag[r] = 1;
+    }
+}
+
+// New initial data for one run: time 0, dt its fixed dt or 0, primed by its next step.
+__global__ void k_restart_run(const RunDev* __restrict__ runs, WorkBuf W, int run)
+{
+    W.dt[run] = runs[run].dt_fixed > 0.0 ? runs[run].dt_fixed : 0.0;
+    W.time[run] = 0.0;
+    W.prime_flag[run] = 1;
+}
+
+hipError_t launch_remap_runs(const RunDev* runs, const WorkBuf& W, int n_old, int n_new, hipStream_t s)
+{
+    if (n_old > 0) k_save_run_scalars<<<(unsigned)((n_old + 255) / 256), 256, 0, s>>>(W, n_old);
+    if (n_new > 0) k_load_run_scalars<<<(unsigned)((n_new + 255) / 256), 256, 0, s>>>(runs, W, n_new);
+    return hipGetLastError();
+}
+
+hipError_t launch_restart_run(const RunDev* runs, const WorkBuf& W, int run, hipStream_t s)
+{
+    k_restart_run<<<1, 1, 0, s>>>(runs, W, run);
+    return hipGetLastError();
+}
+
 hipError_t launch_move_runs(const StateBuf& A, const StateBuf& B, const int64_t* tab, int n_keep, int64_t total,
                             int mixed, int mhd, hipStream_t s)
 {
@@ -1429,8 +1542,9 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n)
 // ------------------------------------------------------------------------------------------------
 static inline int nblk_xcd(int ncell) { return ((ncell + 7) / 8) * 8; }
 
-// Blocks of a pair-pass launch: one wave per supercell (persistent_waves 0), or a persistent grid of
-// that many waves (a multiple of 8) striding over the supercells when there are more cells than waves.
+// Blocks of a chunked pair-pass launch: one wave per supercell (persistent_waves 0), or a persistent
+// grid of that many waves (a multiple of 8) striding over the supercells when there are more cells
+// than waves. The whole-cell passes and the 125-cell stencil launch one wave per supercell.
 static inline int pair_grid(const LaunchCtx& c, int nblk)
 {
     return (c.persistent_waves > 0 && c.persistent_waves < nblk) ? c.persistent_waves : nblk;
@@ -1475,11 +1589,11 @@ hipError_t launch_scatter_gather(const LaunchCtx& c, const StateBuf& Snew)
 #define SPH_COMMA ,
 // Instantiate a pair kernel for (precision, MHD fields, smoothing kernel): KT = -1 reads the kernel
 // type per run (batches that mix kernel types).
-#define SPH_PAIR_DISPATCH(KERNEL, SIZE, EXTRA, ...)                                                 \
+#define SPH_PAIR_DISPATCH(KERNEL, GRIDN, SIZE, EXTRA, ...)                                          \
     do {                                                                                            \
         auto go = [&](auto mixed, auto mhd, auto kt) {                                              \
             KERNEL<SIZE, decltype(mixed)::value, decltype(mhd)::value, decltype(kt)::value EXTRA>    \
-                <<<pair_grid(c, nblk), 64, 0, c.stream>>>(__VA_ARGS__);                              \
+                <<<(GRIDN), 64, 0, c.stream>>>(__VA_ARGS__);                                         \
         };                                                                                          \
         auto kt_go = [&](auto mixed, auto mhd) {                                                    \
             switch (c.kern) {                                                                       \
@@ -1501,25 +1615,42 @@ hipError_t launch_scatter_gather(const LaunchCtx& c, const StateBuf& Snew)
 template <int CH, bool BAL>
 static void density_ch(const LaunchCtx& c, int nblk)
 {
-    SPH_PAIR_DISPATCH(k_density, CH, SPH_COMMA BAL, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
+    const int grid = pair_grid(c, nblk);
+    if (grid < nblk)
+        SPH_PAIR_DISPATCH(k_density, grid, CH, SPH_COMMA BAL SPH_COMMA 1 SPH_COMMA true, c.runs, c.nruns, c.ncell_tot,
+                          nblk, c.S, c.W, c.kmax, c.balance);
+    else
+        SPH_PAIR_DISPATCH(k_density, nblk, CH, SPH_COMMA BAL, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax,
+                          c.balance);
+}
+
+// supercells finer than H somewhere in the context: the chunked lane loop over the 125-cell stencil
+template <int CH>
+static void density_fine_ch(const LaunchCtx& c, int nblk)
+{
+    SPH_PAIR_DISPATCH(k_density, nblk, CH, SPH_COMMA false SPH_COMMA 2, c.runs, c.nruns, c.ncell_tot, nblk, c.S,
+                      c.W, c.kmax, 0);
 }
 
 template <int CAPD>
 static void density_cell_ch(const LaunchCtx& c, int nblk)
 {
-    SPH_PAIR_DISPATCH(k_density_cell, CAPD, , c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
+    SPH_PAIR_DISPATCH(k_density_cell, nblk, CAPD, , c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
 }
 
 template <int CH>
 static void forces_cell_ch(const LaunchCtx& c, int nblk)
 {
-    SPH_PAIR_DISPATCH(k_forces_cell, CH, , c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
+    SPH_PAIR_DISPATCH(k_forces_cell, nblk, CH, , c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
 }
 
 hipError_t launch_density(const LaunchCtx& c)
 {
     const int nblk = nblk_xcd(c.ncell_tot);
-    if (c.balance == 2) {
+    if (c.rmax == 2) {
+        if (c.density_chunk >= 256) density_fine_ch<256>(c, nblk);
+        else density_fine_ch<128>(c, nblk);
+    } else if (c.balance == 2) {
         density_cell_ch<512>(c, nblk);
     } else if (c.density_chunk >= 256) {
         if (c.balance == 1) density_ch<256, true>(c, nblk);
@@ -1534,13 +1665,29 @@ hipError_t launch_density(const LaunchCtx& c)
 template <int CH, int BAL>
 static void forces_ch(const LaunchCtx& c, int nblk)
 {
-    SPH_PAIR_DISPATCH(k_forces, CH, SPH_COMMA BAL, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
+    const int grid = pair_grid(c, nblk);
+    if (grid < nblk)
+        SPH_PAIR_DISPATCH(k_forces, grid, CH, SPH_COMMA BAL SPH_COMMA 1 SPH_COMMA true, c.runs, c.nruns, c.ncell_tot,
+                          nblk, c.S, c.W, c.kmax, c.balance);
+    else
+        SPH_PAIR_DISPATCH(k_forces, nblk, CH, SPH_COMMA BAL, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax,
+                          c.balance);
+}
+
+template <int CH>
+static void forces_fine_ch(const LaunchCtx& c, int nblk)
+{
+    SPH_PAIR_DISPATCH(k_forces, nblk, CH, SPH_COMMA 0 SPH_COMMA 2, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W,
+                      c.kmax, 0);
 }
 
 hipError_t launch_forces(const LaunchCtx& c)
 {
     const int nblk = nblk_xcd(c.ncell_tot);
-    if (c.balance == 2) {
+    if (c.rmax == 2) {
+        if (c.force_chunk >= 128) forces_fine_ch<128>(c, nblk);
+        else forces_fine_ch<64>(c, nblk);
+    } else if (c.balance == 2) {
         forces_cell_ch<128>(c, nblk);
     } else if (c.force_chunk >= 128) {
         if (c.balance == 1) forces_ch<128, 1>(c, nblk);

@@ -19,6 +19,8 @@ struct LaunchCtx {
     int kmax;     // max lanes per target in the pair passes
     int balance;  // share a target's accepted pairs among its K lanes
     int kern;     // smoothing kernel of every run (SPH_KERNEL_*), or -1 when runs differ
+    int rmax;     // largest stencil reach of the runs: 1 (supercells of edge >= H) or 2 (edge >= H/2; pair
+                  // passes take the chunked lane loop, pair_balance 0)
     int persistent_waves;  // pair passes: 0 = one wave per supercell; else a persistent grid of this many
                            // waves (a multiple of 8) walking the supercells at a stride of the grid
     StateBuf S;          // current buffer set
@@ -29,6 +31,11 @@ struct LaunchCtx {
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 // Copy the state of n_keep runs from A to B: run k's particles [src, src + count) to [dst, dst + count),
 // with (dst, src, count) at tab[3k..3k+2] on the device, dst ascending from 0 and runs contiguous.
+// Per-run dt, time and priming flag of a new run set of n_new runs from the n_old old ones (W.run_src:
+// the old run each new run continues, or -1 for a fresh run); runs is the new run table.
+hipError_t launch_remap_runs(const RunDev* runs, const WorkBuf& W, int n_old, int n_new, hipStream_t s);
+// New initial data for one run: time 0, dt its fixed dt or 0, primed by its next step.
+hipError_t launch_restart_run(const RunDev* runs, const WorkBuf& W, int run, hipStream_t s);
 hipError_t launch_move_runs(const StateBuf& A, const StateBuf& B, const int64_t* tab, int n_keep, int64_t total,
                             int mixed, int mhd, hipStream_t s);
 hipError_t launch_drift_bin(const LaunchCtx& c, double dt_host, int do_drift);

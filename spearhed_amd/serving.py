@@ -120,6 +120,7 @@ class ServeStats:
     move_ms: List[float] = field(default_factory=list)     # sph_pool_set_runs
     fill_ms: List[float] = field(default_factory=list)     # uploads / device fills of admitted runs
     step_wall_ms: List[float] = field(default_factory=list)  # host launch to completion, per step
+    harvest_ms: List[float] = field(default_factory=list)    # responses turned into results
 
     def summary(self) -> dict:
         s = np.array(self.step_ms) if self.step_ms else np.zeros(1)
@@ -137,6 +138,8 @@ class ServeStats:
             "retire_ms_total": float(np.sum(self.retire_ms)) if self.retire_ms else 0.0,
             "move_ms_total": float(np.sum(self.move_ms)) if self.move_ms else 0.0,
             "fill_ms_total": float(np.sum(self.fill_ms)) if self.fill_ms else 0.0,
+            "harvest_ms_total": float(np.sum(self.harvest_ms)) if self.harvest_ms else 0.0,
+            "admission_ms_p50": float(np.percentile(self.admit_ms, 50)) if self.admit_ms else 0.0,
         }
 
 
@@ -170,13 +173,19 @@ class Server:
         self.results: dict = {}
         self._t0 = None
         self.response_fields = tuple(response_fields) if response_fields else None
-        self._ring = None
-        self._pending = []  # (resident, arrays, conserved, finished time) awaiting a synchronisation
+        # Responses are copied out into two pinned rings used in turn by retiring rounds: a round's
+        # copies complete at the synchronisation after its step and are harvested during the next
+        # step, while the next retiring round fills the other ring.
+        self._rings = []
+        self._turn = 0
+        self._pending = []  # (ring, resident, arrays, conserved, finished time): copies enqueued
+        self._ready = []    # the same, after a synchronisation: safe to read
         if self.response_fields:
             from .stepper import RESPONSE_FIELDS, PinnedRing
             per = sum(np.dtype(RESPONSE_FIELDS[f][1]).itemsize * RESPONSE_FIELDS[f][2]
                       for f in self.response_fields)
-            self._ring = PinnedRing(self.max_particles * per + 512 * (len(self.response_fields) + 1) * 256)
+            ring_bytes = self.max_particles * per + 512 * (len(self.response_fields) + 1) * 256
+            self._rings = [PinnedRing(ring_bytes), PinnedRing(ring_bytes)]
 
     # -- bookkeeping -------------------------------------------------------------------------------
     def run_bytes(self, req: Request) -> int:
@@ -203,9 +212,12 @@ class Server:
         t_r = time.perf_counter()
         if self.response_fields:
             from .stepper import download_async
+            ring = self._turn
+            if finished:
+                self._turn ^= 1
             for i in finished:
-                arrays, cons = download_async(self.pool, i, self.response_fields, self._ring)
-                self._pending.append((self.resident[i], arrays, cons, now))
+                arrays, cons = download_async(self.pool, i, self.response_fields, self._rings[ring])
+                self._pending.append((ring, self.resident[i], arrays, cons, now))
         else:
             cons = self.pool.conserved() if finished else []
             for i in finished:
@@ -240,14 +252,25 @@ class Server:
                 self.pool.init_run(j, q.blocks, seed=q.seed)
         self.stats.fill_ms.append((time.perf_counter() - t_f) * 1e3)
 
+    def _synced(self):
+        """The pool's stream has been synchronised: every copy-out enqueued so far is complete."""
+        self._ready += self._pending
+        self._pending = []
+
     def _harvest(self):
-        """Turn completed asynchronous copy-outs into results (the stream has been synchronised)."""
-        for r, arrays, cons, fin in self._pending:
+        """Turn completed asynchronous copy-outs into results and free their rings."""
+        if not self._ready:
+            return
+        t0 = time.perf_counter()
+        used = set()
+        for ring, r, arrays, cons, fin in self._ready:
             self.results[r.req.rid] = Result(r.req.rid, None, cons.as_dict(), r.done, r.req.arrival, r.admitted,
                                              fin, fields={k: v.copy() for k, v in arrays.items()})
-        self._pending = []
-        if self._ring is not None:
-            self._ring.reset()
+            used.add(ring)
+        self._ready = []
+        for ring in used:
+            self._rings[ring].reset()
+        self.stats.harvest_ms.append((time.perf_counter() - t0) * 1e3)
 
     def serve(self, requests: Optional[List[Request]] = None, max_steps: int = 1 << 30,
               realtime: bool = True) -> dict:
@@ -267,6 +290,9 @@ class Server:
             self._retire_and_admit(self.now() if realtime else float("inf"))
             self.stats.admit_ms.append((time.perf_counter() - t_adm) * 1e3)
             if not self.resident:
+                self.pool.sync()
+                self._synced()
+                self._harvest()
                 if self.queue:  # idle until the next arrival
                     time.sleep(max(0.0, min(self.queue[0][0] - self.now(), 0.01)))
                 continue
@@ -274,6 +300,7 @@ class Server:
             self.pool.step(1)
             self._harvest()  # responses completed by the last synchronisation: copied while the GPU steps
             self.pool.sync()
+            self._synced()
             self.stats.step_ms.append((time.perf_counter() - t1) * 1e3)
             self.stats.batch_runs.append(len(self.resident))
             self.stats.batch_particles.append(sum(r.req.n for r in self.resident))
@@ -282,6 +309,7 @@ class Server:
             n_steps += 1
         self._retire_and_admit(float("-inf"))  # retire the last runs, admit nothing
         self.pool.sync()
+        self._synced()
         self._harvest()
         if profiled:
             prof = self.pool.profile_end()
@@ -292,8 +320,8 @@ class Server:
 
     def close(self):
         self.pool.close()
-        if self._ring is not None:
-            self._ring.close()
+        for ring in self._rings:
+            ring.close()
 
 
 def poisson_arrivals(n: int, rate_per_s: float, seed: int = 0) -> np.ndarray:

@@ -139,6 +139,16 @@ class Context:
     def profile_begin(self, max_steps: int):
         check(self.lib.sph_profile_begin(self.h, max_steps), self.h, "sph_profile_begin")
 
+    def mark(self) -> int:
+        """Ticket of the current point of the context's stream (sph_mark)."""
+        t = C.c_uint64()
+        check(self.lib.sph_mark(self.h, C.byref(t)), self.h, "sph_mark")
+        return t.value
+
+    def wait_mark(self, ticket: int):
+        """Block until the stream has passed the point of `ticket` (sph_mark_wait)."""
+        check(self.lib.sph_mark_wait(self.h, int(ticket)), self.h, "sph_mark_wait")
+
     def profile_end(self):
         p = capi.Profile()
         steps = (C.c_double * 100000)()

@@ -28,7 +28,7 @@ class OrcGrid(C.Structure):
         ("n", C.c_int32 * 3), ("ncell", C.c_int32), ("support", C.c_int32),
         ("lo", C.c_double * 3), ("L", C.c_double * 3), ("s", C.c_double * 3), ("inv_s", C.c_double * 3),
         ("lo32", C.c_float * 3), ("L32", C.c_float * 3), ("s32", C.c_float * 3), ("inv_s32", C.c_float * 3),
-        ("H", C.c_float), ("H2", C.c_float), ("Hd", C.c_double), ("H2d", C.c_double),
+        ("H", C.c_float), ("H2", C.c_float), ("Hd", C.c_double), ("H2d", C.c_double), ("reach", C.c_int32),
     ]
 
 

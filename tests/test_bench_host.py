@@ -59,3 +59,15 @@ def test_measured_traffic_only_for_the_profiled_workload():
     assert bench.measured_traffic("k_forces", rec["particles"], other) is None
     assert bench.measured_traffic("k_forces", rec["particles"] + 1, rec["tuning"]) is None
     assert bench.measured_traffic("k_density", rec["particles"], rec["tuning"]) is None
+
+
+def test_step_roofline_sums_stage_bounds():
+    """At 2M particles and 60 pairs each: the force pass is FP32-bound (13.6 GFLOP at 157.3 TFLOP/s,
+    0.087 ms, against 0.033 ms for its bytes), the HBM-bound stages cost bytes / 8 TB/s."""
+    n, pairs = 2_097_152, 60 * 2_097_152
+    r = bench.step_roofline(n, pairs, 1.85)
+    st = r["stage_sol_ms"]
+    assert st["forces"] == pytest.approx((108 * pairs + 80 * n) / 157.3e12 * 1e3)
+    assert st["drift_bin"] == pytest.approx(70 * n / 8e12 * 1e3)
+    assert r["sol_ms"] == pytest.approx(sum(st.values()))
+    assert r["frac"] == pytest.approx(r["sol_ms"] / 1.85) and 0 < r["frac"] < 1

@@ -148,3 +148,52 @@ def test_turbulence_2m_ten_steps_against_the_oracle():
     assert np.array_equal(g.nnb, r.nnb[kr]), "neighbour counts of the developed state differ"
     rel = np.abs(g.rho.astype(np.float64) - r.rho[kr]) / r.rho[kr]
     assert float(rel.max()) <= 1e-5
+
+
+def test_ensemble_256_autotuned_graph_50_steps_against_the_oracle():
+    """The ensemble configuration as benchmarked: 256 Kelvin-Helmholtz runs in one context, launch
+    shape picked by the autotuner, primed, then 50 steps (the configuration's count) by hipGraph
+    replay. Sampled runs against the FP64 oracle's run of the same run alone, by particle id:
+    density <= 1e-5 relative, velocity and B <= 1e-5 of the field's largest magnitude, position
+    <= 1e-5 smoothing lengths, time and energy <= 1e-6 (the FP32 oracle stays within 1.8e-6 of the
+    FP64 one over these 50 steps)."""
+    runs = ic.kh_ensemble(256, seed=4)
+    sizes = [p.n for _, p in runs]
+    steps = 50
+    sample = [0, int(np.argmin(sizes)), int(np.argmax(sizes)), 97, 200]
+    print(f"ensemble autotuned + graph: runs={len(runs)} particles={sum(sizes)} steps={steps}", flush=True)
+    with Context([c for c, _ in runs], sizes, precision="mixed") as ctx:
+        for r, (_, ps) in enumerate(runs):
+            ctx.upload(r, ps)
+        ctx.prime()
+        tuning = ctx.autotune(2)
+        print(f"tuning {tuning}", flush=True)
+        ctx.prime()
+        ctx.graph_capture()
+        ctx.graph_launch(steps)
+        ctx.sync()
+        cons = ctx.conserved()
+        got = {r: ctx.download(r).sorted_by_id() for r in sample}
+    for r in sample:
+        cfg, ps = runs[r]
+        o = oracle_ref.OracleRun(cfg, ps, "f64")
+        o.prime()
+        oc = o.step(steps)
+        k = o.by_id()
+        g = got[r]
+        err = {"rho": float(np.max(np.abs(g.rho.astype(np.float64) - o.rho[k]) / o.rho[k]))}
+        for f in ("vel", "B"):
+            ref = getattr(o, f)[:, k]
+            top = float(np.max(np.linalg.norm(ref, axis=0)))
+            err[f] = float(np.max(np.abs(getattr(g, f).astype(np.float64) - ref))) / top
+        L = np.array(cfg.box_len)[:, None]
+        d = g.pos - o.pos[:, k]
+        d -= L * np.round(d / L)
+        err["pos/h"] = float(np.max(np.abs(d)) / cfg.h0)
+        err["time"] = abs(cons[r]["time"] - oc["time"]) / oc["time"]
+        err["energy"] = abs(cons[r]["energy"] - oc["energy"]) / abs(oc["energy"])
+        print(f"run {r} n={ps.n}: " + ", ".join(f"{a} {b:.2e}" for a, b in err.items()), flush=True)
+        for f in ("rho", "vel", "B", "pos/h"):
+            assert err[f] <= 1e-5, (r, f, err[f])
+        for f in ("time", "energy"):
+            assert err[f] <= 1e-6, (r, f, err[f])

@@ -23,10 +23,21 @@ from spearhed_amd.stepper import Context
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-# Relative bounds, about 30-100x what was measured (time 4e-8, total energy 2.3e-9, kinetic and thermal
-# energy 4e-10, magnetic energy 2.3e-8, div-B error 1.1e-6, maps 3e-8 of their maximum): at this
-# stage of the vortex the FP32 summation-order differences do not grow into trajectory differences.
-REL = {"time": 1e-6, "energy": 1e-7, "e_kin": 1e-7, "e_th": 1e-7, "e_mag": 1e-6, "divb_err": 1e-4, "maps": 1e-6}
+# Relative bounds. Mixed precision, 100 steps: about 30-100x what was measured (time 4e-8, total
+# energy 2.3e-9, kinetic and thermal energy 4e-10, magnetic energy 2.3e-8, div-B error 1.1e-6, maps
+# 3e-8 of their maximum): the FP32 summation-order differences do not grow into trajectory
+# differences when positions are FP64. Plain FP32, 200 steps: positions themselves are FP32, so a
+# rounding difference can move a particle by an ulp of its absolute coordinate, and both runs differ
+# from each other as much as FP32 arithmetic differs from FP64: the GPU against the FP32 oracle
+# measured time 2.7e-6, energies <= 5.1e-8, div-B error 8.2e-7, maps <= 1.07e-4 of their maximum,
+# while the FP32 oracle against the FP64 oracle measured time 2.5e-7, energies <= 2.6e-8, div-B error
+# 8.4e-6, maps <= 1.45e-4. The bounds are about 3-4x the larger of the two.
+REL = {
+    "mixed": {"time": 1e-6, "energy": 1e-7, "e_kin": 1e-7, "e_th": 1e-7, "e_mag": 1e-6, "divb_err": 1e-4,
+              "maps": 1e-6},
+    "fp32": {"time": 1e-5, "energy": 1e-7, "e_kin": 3e-7, "e_th": 3e-7, "e_mag": 3e-7, "divb_err": 3e-5,
+             "maps": 5e-4},
+}
 
 
 @pytest.mark.parametrize("fixture,precision,steps", [("ot_fp32_200", "fp32", 200), ("ot_full_100", "mixed", 100)])
@@ -48,11 +59,12 @@ def test_orszag_tang_full_size_against_the_oracle(fixture, precision, steps):
     maps = {k: float(np.max(np.abs(st[k] - z[k])) / np.max(np.abs(z[k]))) for k in ("map_rho", "map_bx", "map_by")}
     print(f"OT: t {c['time']:.9f} vs {float(z['time']):.9f}; energy {c['energy']:.12e} vs {float(z['energy']):.12e}; "
           f"relative differences {rel}; map differences (of the map's max) {maps}", flush=True)
-    assert abs(c["time"] - float(z["time"])) <= REL["time"] * float(z["time"])
-    assert abs(c["energy"] - float(z["energy"])) <= REL["energy"] * abs(float(z["energy"]))
+    rel_b = REL[precision]
+    assert abs(c["time"] - float(z["time"])) <= rel_b["time"] * float(z["time"])
+    assert abs(c["energy"] - float(z["energy"])) <= rel_b["energy"] * abs(float(z["energy"]))
     assert c["mass"] == pytest.approx(float(z["mass"]), rel=1e-12)
     for k in ("e_kin", "e_th", "e_mag", "divb_err"):
-        assert rel[k] <= REL[k], (k, rel)
+        assert rel[k] <= rel_b[k], (k, rel)
     for k, v in maps.items():
-        assert v <= REL["maps"], (k, maps)
+        assert v <= rel_b["maps"], (k, maps)
     assert st["divb_err"] < 1e-2  # the cleaning keeps |div B| h / |B| small

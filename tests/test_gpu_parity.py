@@ -391,3 +391,47 @@ def test_persistent_grid_batch_matches_oracle_and_alone(wpc):
             alone = ctx.download(0).sorted_by_id()
         for f in ("pos", "vel", "u", "rho", "B", "dvdt", "nnb"):
             assert np.array_equal(getattr(alone, f), getattr(stepped[r], f)), (r, f)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("shape", [dict(density_chunk=128, force_chunk=64, waves_per_cu=0),
+                                   dict(density_chunk=256, force_chunk=128, waves_per_cu=0),
+                                   dict(density_chunk=128, force_chunk=64, waves_per_cu=1)])
+def test_half_h_supercells_parity(prec, shape):
+    """Supercells of edge >= H/2 (cell_factor 0.5, the 125-cell stencil): neighbour counts bit-exact and
+    density / forces within tolerance against the oracle on the same grid."""
+    for name, (cfg, ps) in (("turbulence", _turb(16, 19)), ("sod_walls", ic.sod_tube(seed=6, nx_left=32, cross=4))):
+        cfg.cell_factor = 0.5
+        print(f"half-H {name} prec {prec} {shape} n={ps.n}", flush=True)
+        g = gpu_stages(cfg, ps, GPREC[prec], tuning=dict(shape, max_lanes_per_target=64, pair_balance=0))
+        o = oracle_stages(cfg, ps, prec)
+        assert oracle_ref.grid(cfg).reach == 2
+        compare_forces(g, o, bool(cfg.mhd))
+
+
+def test_half_h_and_full_h_runs_batched_equal_alone():
+    """A context mixing a run on supercells of edge >= H/2 with runs on edge >= H takes the 125-cell
+    kernels for all; each run still equals itself alone, bit for bit, after three steps, and the
+    whole-cell or balanced pair loops are refused while a finer grid is present."""
+    fine = ic.kelvin_helmholtz(5000, 81)
+    fine[0].cell_factor = 0.5
+    runs = [fine, ic.kelvin_helmholtz(6000, 82), ic.mhd_turbulence(n=14, seed=83)]
+    cfgs, n = [c for c, _ in runs], [p.n for _, p in runs]
+    with Context(cfgs, n, precision="mixed") as ctx:
+        with pytest.raises(capi.SphError):
+            ctx.set_tuning(pair_balance=1)
+        for r, (_, p) in enumerate(runs):
+            ctx.upload(r, p)
+        ctx.prime()
+        ctx.step(3)
+        ctx.sync()
+        got = [ctx.download(r).sorted_by_id() for r in range(len(runs))]
+    for (cfg, ps), g in zip(runs, got):
+        with Context([cfg], [ps.n], precision="mixed") as ctx:
+            ctx.upload(0, ps)
+            ctx.prime()
+            ctx.step(3)
+            ctx.sync()
+            a = ctx.download(0).sorted_by_id()
+        for f in ("pos", "vel", "u", "rho", "B", "dvdt", "dbdt", "nnb"):
+            assert np.array_equal(getattr(a, f), getattr(g, f)), (cfg.cell_factor, f)

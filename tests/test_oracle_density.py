@@ -75,16 +75,20 @@ def test_all_pairs_counted_once(oracle_lib):
 
 @pytest.mark.parametrize("prec", ["plain", "f64"])
 @pytest.mark.parametrize("box", [(1.0, 0.3, 0.55), (0.21, 0.9, 0.42)])
-def test_binned_counts_equal_brute_force(oracle_lib, prec, box):
-    """Periodic boxes with 1, 2 and several supercells per axis: the 27-cell stencil with periodic
-    images finds exactly the pairs a brute-force search over all images finds."""
+@pytest.mark.parametrize("cell_factor", [1.0, 0.5])
+def test_binned_counts_equal_brute_force(oracle_lib, prec, box, cell_factor):
+    """Periodic boxes with 1, 2 and several supercells per axis: the stencil with periodic images
+    (27 supercells of edge >= H, or 125 of edge >= H/2) finds exactly the pairs a brute-force search
+    over all images finds."""
     rng = np.random.default_rng(11)
     n = 600
     pos = rng.uniform(0, 1, size=(3, n)) * np.array(box)[:, None]
     h = 0.1
-    cfg = capi.make_config(box_len=box, periodic=(1, 1, 1), kernel=capi.KERNEL_WENDLAND, h0=h)
+    cfg = capi.make_config(box_len=box, periodic=(1, 1, 1), kernel=capi.KERNEL_WENDLAND, h0=h,
+                           cell_factor=cell_factor)
     g = O.grid(cfg)
-    assert min(g.n) <= 2  # exercises the duplicated-image path
+    assert g.reach == (2 if cell_factor < 1 else 1)
+    assert min(g.n) <= 2 * g.reach  # exercises the duplicated-image path
     ps = _ps(pos, 1.0, h)
     a = O.OracleRun(cfg, ps, prec)
     a.resort()
@@ -94,6 +98,42 @@ def test_binned_counts_equal_brute_force(oracle_lib, prec, box):
     brute = b.brute_counts()
     assert np.array_equal(binned, brute)
     assert binned.mean() > 5
+
+
+@pytest.mark.parametrize("prec,rtol", [("f64", 1e-12), ("plain", 2e-6)])
+def test_half_h_supercells_equal_the_27_cell_stencil(oracle_lib, prec, rtol):
+    """Supercells of edge >= H/2 (125-cell stencil) against edge >= H (27 cells) on the same
+    particles: in absolute coordinates the same pairs pass the cull (counts bit-exact, and equal to
+    brute force); density and forces agree to the rounding of a different summation order."""
+    cfg, ps = ic.mhd_turbulence(n=14, seed=9)
+    out = {}
+    for cf in (1.0, 0.5):
+        cfg.cell_factor = cf
+        g = O.grid(cfg)
+        o = O.OracleRun(cfg, ps, prec, scales=True)
+        o.resort()
+        o.density()
+        o.forces()
+        out[cf] = (o, o.by_id(), g)
+    (a, ka, ga), (b, kb, gb) = out[1.0], out[0.5]
+    assert gb.reach == 2 and ga.reach == 1 and gb.ncell >= 8 * ga.ncell
+    assert np.array_equal(a.nnb[ka], b.nnb[kb])
+    if prec == "f64":
+        assert np.array_equal(b.nnb[kb], O.OracleRun(cfg, ps, prec).brute_counts())
+    assert np.max(np.abs(a.rho[ka] - b.rho[kb]) / a.rho[ka]) <= rtol
+    assert np.max(np.abs(a.dvdt[:, ka] - b.dvdt[:, kb]) / a.sc_a[ka]) <= rtol
+    assert np.max(np.abs(a.dbdt[:, ka] - b.dbdt[:, kb]) / a.sc_db[ka]) <= rtol
+
+
+def test_cell_factor_mapping(oracle_lib):
+    """Unset (<= 0) means 1; below 1/2 means 1/2; edges at least cell_factor * H."""
+    cfg = capi.make_config(h0=0.05)
+    H = 0.1
+    for cf, reach, n in ((0.0, 1, 9), (1.0, 1, 9), (0.75, 2, 13), (0.5, 2, 19), (0.3, 2, 19), (2.0, 1, 4)):
+        cfg.cell_factor = cf
+        g = O.grid(cfg)
+        assert (g.reach, g.n[0]) == (reach, n), cf
+        assert g.s[0] * (2 if reach == 2 else 1) >= H
 
 
 @pytest.mark.parametrize("prec", ["plain", "mixed"])
