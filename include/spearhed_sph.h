@@ -293,6 +293,43 @@ int sph_profile_end(sph_ctx* ctx, sph_profile* out, double* step_ms);
  * reference that HBM counter readings (FETCH_SIZE / WRITE_SIZE) are calibrated against. */
 int sph_probe_copy(int64_t bytes, int32_t reps, double* ms_per_copy);
 
+/* ---- density without device allocation -------------------------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc) and so cannot create a
+ * context: the resort and density pass of sph_stage_resort + sph_stage_density, run on one run whose
+ * particles the caller holds in device memory, with the library's own working memory instead of a
+ * context's. Results equal sph_upload + sph_stage_density + sph_download of the same particles with
+ * ids 0..n-1 in input order (particles of a supercell are ordered by input index), bit for bit.
+ *
+ * The working memory is one array of at most SPH_NOALLOC_SCRATCH_BYTES per GPU, held from the first
+ * call on that GPU until the process ends; the call allocates nothing. Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams or host threads) race and must be
+ * serialised by the caller. Sizes beyond SPH_NOALLOC_MAX_PARTICLES particles or
+ * SPH_NOALLOC_MAX_CELLS supercells are rejected with SPH_E_CAPACITY before anything is launched. */
+#define SPH_NOALLOC_MAX_PARTICLES 262144
+#define SPH_NOALLOC_MAX_CELLS 262144
+#define SPH_NOALLOC_SCRATCH_BYTES (48ll << 20)
+
+/* Device columns of one run (device pointers, caller-owned, read only). */
+typedef struct sph_device_soa {
+    int64_t n;
+    const uint8_t* mask;          /* multiMask, 1 = live; NULL: every particle live */
+    const uint8_t* species;       /* SPH_SPECIES_*; NULL: every particle SPH_SPECIES_DEFAULT */
+    const double* pos[3];         /* absolute positions, FP64: required in SPH_PREC_MIXED */
+    const float* pos32[3];        /* absolute positions, FP32: required in SPH_PREC_FP32 */
+    const float* mass;            /* required */
+    const float* smoothing_length; /* required; finite and > 0 (not checked: it stays on the device) */
+} sph_device_soa;
+
+/* Density and neighbour counts of the particles in `in`, written to the device arrays density_out
+ * and n_neighbours_out (n entries each, input order; 0 for particles that are not live Default
+ * particles), asynchronously on `stream` (a hipStream_t; NULL = the null stream). errbuf (may be
+ * NULL) receives the message of a failure. The smoothing kernel, radius and box come from cfg; the
+ * launch shape is a fresh context's default tuning. */
+int sph_density_noalloc(const sph_run_config* cfg, int32_t precision, const sph_device_soa* in,
+                        float* density_out, uint32_t* n_neighbours_out, void* stream, char* errbuf,
+                        int32_t errbuf_len);
+
 /* ---- autotuner ------------------------------------------------------------------------------ */
 /* Time n_steps steps for each candidate tuning on the context's current data (state restored
  * afterwards) and keep the fastest. Writes the chosen tuning to *best if non-NULL. The state is

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libspearhed_sph.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
-SOURCES = ["sph_kernels.hip", "sph_context.cpp", "sph_probe.hip", "sph_setup.hip"]
+SOURCES = ["sph_kernels.hip", "sph_context.cpp", "sph_probe.hip", "sph_setup.hip", "sph_noalloc.hip"]
 ARCH = "gfx950"
 
 # -packed-fp32-ops off: plain float code must not be turned into packed FP32 instructions.

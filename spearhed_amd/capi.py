@@ -193,6 +193,23 @@ class Profile(C.Structure):
                 "step_ms_p99": self.step_ms_p99}
 
 
+NOALLOC_MAX_PARTICLES = 262144
+NOALLOC_MAX_CELLS = 262144
+
+
+class DeviceSoA(C.Structure):
+    """sph_device_soa: device pointers (as integers) of one run's columns for sph_density_noalloc."""
+    _fields_ = [
+        ("n", C.c_int64),
+        ("mask", C.c_void_p),
+        ("species", C.c_void_p),
+        ("pos", C.c_void_p * 3),
+        ("pos32", C.c_void_p * 3),
+        ("mass", C.c_void_p),
+        ("smoothing_length", C.c_void_p),
+    ]
+
+
 # name -> (restype, argtypes); exactly the entry points of include/spearhed_sph.h
 _CTX = C.c_void_p
 PROTOTYPES = {
@@ -239,6 +256,8 @@ PROTOTYPES = {
     "sph_profile_end": (C.c_int, [_CTX, C.POINTER(Profile), C.POINTER(C.c_double)]),
     "sph_autotune": (C.c_int, [_CTX, C.c_int32, C.POINTER(Tuning)]),
     "sph_probe_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "sph_density_noalloc": (C.c_int, [C.POINTER(RunConfig), C.c_int32, C.POINTER(DeviceSoA), C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_char_p, C.c_int32]),
 }
 
 _lib = None

@@ -573,6 +573,17 @@ int open_stream(sph_ctx* c, void* stream)
 
 }  // namespace
 
+namespace sph {
+// The grid and constants of one run from its configuration (for entry points without a context).
+bool make_run_config(const sph_run_config& c, RunDev& R, char* why, int why_len)
+{
+    std::string w;
+    const bool ok = make_run(c, R, w);
+    if (!ok && why && why_len > 0) snprintf(why, (size_t)why_len, "%s", w.c_str());
+    return ok;
+}
+}  // namespace sph
+
 // =================================================================================================
 // C ABI
 // =================================================================================================

@@ -87,6 +87,7 @@ struct WorkBuf {
     int64_t* move_tab;   // per kept run when a pool's run set changes: destination start, source start, count
     int32_t* run_src;    // per run of a new run set: the old run it continues, or -1 for a new run
     double* run_tmp;     // per old run: its dt, time and priming flag, saved while the run set changes
+    int32_t* perm;       // optional (may be null): the resort writes each sorted slot's source index here
 };
 
 constexpr float kSentinel = 8.507059e37f;  // FLT_MAX / 4: sentinel^2 overflows to +inf, so r2 < H2 fails

@@ -255,6 +255,7 @@ __global__ __launch_bounds__(256) void k_gather(const RunDev* __restrict__ runs,
     for (; mm < ce; ++mm) rank += (W.tmpid[mm] < myid) ? 1u : 0u;
     const uint32_t f = cb + rank;
     if (rank == 0) W.count[c] = 0u;  // histogram reset for the next resort
+    if (W.perm) W.perm[f] = (int32_t)i;
 
     Snew.id[f] = myid;
     const uint8_t live = Sold.mask[i], sp = Sold.species[i];
@@ -614,7 +615,7 @@ __device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ 
     const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
     const float H2 = R.H2;
     const float gam = R.gamma;
-    const bool keep = masks_fit(nt, total, lm);
+    const bool keep = masks_fit(nt, total, lm) && W.nbmask != nullptr;  // no mask table: density only
     MaskWriter mw;
     mw.row = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
     mw.pend = 0;
@@ -1076,7 +1077,7 @@ __global__ __launch_bounds__(64) void k_density_cell(const RunDev* __restrict__ 
     const LaneMap lm = lane_map(nt, kmax);
     const int K = lm.K, k = lm.k;
     const int Lr = (total + K - 1) / K;
-    const bool keep = nt <= lm.tpg && Lr <= 32 * kMaskWords;
+    const bool keep = nt <= lm.tpg && Lr <= 32 * kMaskWords && W.nbmask != nullptr;
     const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
     const float H2 = R.H2;
     const float gam = R.gamma;
