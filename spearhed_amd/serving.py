@@ -11,8 +11,9 @@ All resident runs then advance one step together (one launch per stage for the w
 run keeps its own dt and time, and its summation orders depend only on its own data, so a run's
 result is bit-identical whether it is served alone or in a mix.
 
-The step latency recorded per batched step is its device time (HIP events on the pool's stream);
-the host wall time from launch to completion is kept beside it. p50 / p99 come from those samples.
+The step latency recorded per batched step is its device time (HIP events on the pool's stream); the
+host-side interval between consecutive step completions is kept beside it (one step is kept in
+flight, so the host's admission work overlaps the device's step). p50 / p99 come from those samples.
 """
 from __future__ import annotations
 
@@ -20,6 +21,7 @@ import ctypes as C
 import heapq
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -29,6 +31,9 @@ from . import capi
 from .capi import RunConfig
 from .particles import ParticleSet
 from .stepper import Pool
+
+
+RESPONSE_RINGS = 3  # pinned response rings, each sized for the pool's whole capacity
 
 
 def pool_capacity(budget: int, max_runs: int, cells_per_particle: float, precision: str = "mixed",
@@ -119,7 +124,7 @@ class ServeStats:
     retire_ms: List[float] = field(default_factory=list)   # reductions + downloads of finished runs
     move_ms: List[float] = field(default_factory=list)     # sph_pool_set_runs
     fill_ms: List[float] = field(default_factory=list)     # uploads / device fills of admitted runs
-    step_wall_ms: List[float] = field(default_factory=list)  # host launch to completion, per step
+    step_wall_ms: List[float] = field(default_factory=list)  # host-side interval between step completions
     harvest_ms: List[float] = field(default_factory=list)    # responses turned into results
 
     def summary(self) -> dict:
@@ -173,9 +178,9 @@ class Server:
         self.results: dict = {}
         self._t0 = None
         self.response_fields = tuple(response_fields) if response_fields else None
-        # Responses are copied out into two pinned rings used in turn by retiring rounds: a round's
-        # copies complete at the synchronisation after its step and are harvested during the next
-        # step, while the next retiring round fills the other ring.
+        # Responses are copied out into pinned rings used in turn by retiring rounds: a round's
+        # copies complete with the step behind which they were enqueued and are then copied on into
+        # the results by worker threads, while the next retiring rounds fill the other rings.
         self._rings = []
         self._turn = 0
         self._pending = []  # (ring, resident, arrays, conserved, finished time): copies enqueued
@@ -185,7 +190,11 @@ class Server:
             per = sum(np.dtype(RESPONSE_FIELDS[f][1]).itemsize * RESPONSE_FIELDS[f][2]
                       for f in self.response_fields)
             ring_bytes = self.max_particles * per + 512 * (len(self.response_fields) + 1) * 256
-            self._rings = [PinnedRing(ring_bytes), PinnedRing(ring_bytes)]
+            self._rings = [PinnedRing(ring_bytes) for _ in range(RESPONSE_RINGS)]
+        # responses leave the pinned rings on worker threads (numpy copies release the GIL), so the
+        # loop goes on stepping; a ring is reused only after its copies have finished
+        self._copiers = ThreadPoolExecutor(max_workers=8) if self.response_fields else None
+        self._copies = [[] for _ in self._rings]  # outstanding copy jobs per ring
 
     # -- bookkeeping -------------------------------------------------------------------------------
     def run_bytes(self, req: Request) -> int:
@@ -214,7 +223,8 @@ class Server:
             from .stepper import download_async
             ring = self._turn
             if finished:
-                self._turn ^= 1
+                self._turn = (self._turn + 1) % len(self._rings)
+                self._drain(ring)  # the ring's last copies out must be done before it is refilled
             for i in finished:
                 arrays, cons = download_async(self.pool, i, self.response_fields, self._rings[ring])
                 self._pending.append((ring, self.resident[i], arrays, cons, now))
@@ -258,24 +268,42 @@ class Server:
         self._pending = []
 
     def _harvest(self):
-        """Turn completed asynchronous copy-outs into results and free their rings."""
+        """Turn completed asynchronous copy-outs into results: each response's arrays are copied out of
+        the pinned ring on the worker threads (a Result's fields are complete once the serve call
+        returns, or once the ring has been drained)."""
         if not self._ready:
             return
         t0 = time.perf_counter()
-        used = set()
         for ring, r, arrays, cons, fin in self._ready:
+            fields = {}
+            for k, v in arrays.items():
+                dst = np.empty_like(v)
+                fields[k] = dst
+                self._copies[ring].append(self._copiers.submit(np.copyto, dst, v))
             self.results[r.req.rid] = Result(r.req.rid, None, cons.as_dict(), r.done, r.req.arrival, r.admitted,
-                                             fin, fields={k: v.copy() for k, v in arrays.items()})
-            used.add(ring)
+                                             fin, fields=fields)
         self._ready = []
-        for ring in used:
-            self._rings[ring].reset()
+        self.stats.harvest_ms.append((time.perf_counter() - t0) * 1e3)
+
+    def _drain(self, ring: int):
+        """Wait for the copies out of a ring, then free it for new copy-outs."""
+        t0 = time.perf_counter()
+        for f in self._copies[ring]:
+            f.result()
+        self._copies[ring] = []
+        self._rings[ring].reset()
         self.stats.harvest_ms.append((time.perf_counter() - t0) * 1e3)
 
     def serve(self, requests: Optional[List[Request]] = None, max_steps: int = 1 << 30,
               realtime: bool = True) -> dict:
         """Run until every request has finished. realtime=False admits a request as soon as the
-        queue reaches it (arrival times only order the queue)."""
+        queue reaches it (arrival times only order the queue).
+
+        One step is kept in flight: each round retires and admits runs (copy-outs, run-set change and
+        device fills enqueued behind the step in flight), enqueues the next batched step, and only
+        then waits for the previous one, so that the host's admission work overlaps the device's
+        step. step_ms is each step's device time (HIP events around it); step_wall_ms the host-side
+        interval between consecutive step completions."""
         for q in requests or []:
             self.submit(q)
         # per-step device time from events recorded on the pool's stream (host work between steps,
@@ -285,41 +313,55 @@ class Server:
             self.pool.profile_begin(sum(q[2].steps + 1 for q in self.queue) + 8)
         self._t0 = time.perf_counter()
         n_steps = 0
-        while (self.queue or self.resident) and n_steps < max_steps:
+        inflight = None  # (mark of the step, copy-outs enqueued before it)
+        last_done = None
+        while (self.queue or self.resident or inflight) and n_steps < max_steps:
             t_adm = time.perf_counter()
             self._retire_and_admit(self.now() if realtime else float("inf"))
             self.stats.admit_ms.append((time.perf_counter() - t_adm) * 1e3)
-            if not self.resident:
+            launched = None
+            if self.resident:
+                self.pool.step(1)
+                launched = (self.pool.mark(), self._pending)
+                self._pending = []
+                self.stats.batch_runs.append(len(self.resident))
+                self.stats.batch_particles.append(sum(r.req.n for r in self.resident))
+                for r in self.resident:
+                    r.done += 1
+                n_steps += 1
+            if inflight is not None:
+                self.pool.wait_mark(inflight[0])  # the previous step (and what preceded it) is done
+                now = time.perf_counter()
+                if last_done is not None:
+                    self.stats.step_wall_ms.append((now - last_done) * 1e3)
+                last_done = now
+                self._ready += inflight[1]
+                self._harvest()  # copied while the step just launched runs
+            inflight = launched
+            if inflight is None:
                 self.pool.sync()
                 self._synced()
                 self._harvest()
+                last_done = None
                 if self.queue:  # idle until the next arrival
                     time.sleep(max(0.0, min(self.queue[0][0] - self.now(), 0.01)))
-                continue
-            t1 = time.perf_counter()
-            self.pool.step(1)
-            self._harvest()  # responses completed by the last synchronisation: copied while the GPU steps
-            self.pool.sync()
-            self._synced()
-            self.stats.step_ms.append((time.perf_counter() - t1) * 1e3)
-            self.stats.batch_runs.append(len(self.resident))
-            self.stats.batch_particles.append(sum(r.req.n for r in self.resident))
-            for r in self.resident:
-                r.done += 1
-            n_steps += 1
+        if inflight is not None:  # stopped by max_steps with a step in flight
+            self._pending = inflight[1] + self._pending
         self._retire_and_admit(float("-inf"))  # retire the last runs, admit nothing
         self.pool.sync()
         self._synced()
         self._harvest()
+        for ring in range(len(self._rings)):
+            self._drain(ring)
         if profiled:
             prof = self.pool.profile_end()
-            if len(prof["step_ms"]) == len(self.stats.step_ms):
-                self.stats.step_wall_ms = self.stats.step_ms
-                self.stats.step_ms = list(prof["step_ms"])
+            self.stats.step_ms = list(prof["step_ms"])
         return self.stats.summary()
 
     def close(self):
         self.pool.close()
+        if self._copiers is not None:
+            self._copiers.shutdown(wait=True)
         for ring in self._rings:
             ring.close()
 

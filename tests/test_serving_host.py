@@ -49,6 +49,13 @@ class RecorderPool:
     def sync(self):
         pass
 
+    def mark(self):
+        self.calls.append(("mark",))
+        return len(self.calls)
+
+    def wait_mark(self, ticket):
+        assert 0 < ticket <= len(self.calls)
+
     def download(self, run):
         return ("state", run, self.sizes[run])
 
