@@ -797,6 +797,48 @@ __device__ __forceinline__ void force_pair(const float4& o0, const float4& o1, c
     }
 }
 
+// One target's derivatives, signal speed and div B from its summed pair terms T.
+template <bool MHD>
+__device__ __forceinline__ void force_write(const RunDev& R, WorkBuf& W, const ForceAcc& T, int i, const float4& o1,
+                                            const float4& o2, const float4& o3, const float4& o4, bool mhd,
+                                            bool clean, float borve, float ch)
+{
+    const float rho = o2.w;
+    const float invrho = 1.0f / rho;
+    float ax = T.ax, ay = T.ay, az = T.az;
+    if (MHD && mhd) {
+        const float tcoef = T.Ti - borve * (T.Ti + T.Tj);
+        ax = ax + T.TBx + o4.x * tcoef;
+        ay = ay + T.TBy + o4.y * tcoef;
+        az = az + T.TBz + o4.z * tcoef;
+    }
+    W.dvdt[0][i] = ax;
+    W.dvdt[1][i] = ay;
+    W.dvdt[2][i] = az;
+    W.dudt[i] = o3.y * T.DV + 0.5f * T.AVU;
+    W.vsig[i] = fmaxf(2.0f * o1.w, T.vmax);
+    if (MHD && mhd) {
+        const float divB = -T.DB * invrho;
+        const float divv = -T.DV * invrho;
+        float dbx = -(T.VBx - o4.x * T.DV) * invrho;
+        float dby = -(T.VBy - o4.y * T.DV) * invrho;
+        float dbz = -(T.VBz - o4.z * T.DV) * invrho;
+        float dpsi = 0.0f;
+        if (clean) {
+            dbx -= rho * T.PSx;
+            dby -= rho * T.PSy;
+            dbz -= rho * T.PSz;
+            const float psi = o4.w;
+            dpsi = -ch * ch * divB - psi * R.clean_sigma * ch * o2.y - 0.5f * psi * divv;
+        }
+        W.dbdt[0][i] = dbx;
+        W.dbdt[1][i] = dby;
+        W.dbdt[2][i] = dbz;
+        W.dpsidt[i] = dpsi;
+        W.divb[i] = divB;
+    }
+}
+
 // K partial force sums of a target combined in lane order (deterministic), field by field; then
 // one lane per target writes its derivatives, signal speed and div B.
 template <bool MHD>
@@ -850,42 +892,7 @@ __device__ __forceinline__ void force_finish(const RunDev& R, WorkBuf& W, const 
         T.PSy = s[15];
         T.PSz = s[16];
     }
-    if (k == 0 && tgt) {
-        const float rho = o2.w;
-        const float invrho = 1.0f / rho;
-        float ax = T.ax, ay = T.ay, az = T.az;
-        if (MHD && mhd) {
-            const float tcoef = T.Ti - borve * (T.Ti + T.Tj);
-            ax = ax + T.TBx + o4.x * tcoef;
-            ay = ay + T.TBy + o4.y * tcoef;
-            az = az + T.TBz + o4.z * tcoef;
-        }
-        W.dvdt[0][i] = ax;
-        W.dvdt[1][i] = ay;
-        W.dvdt[2][i] = az;
-        W.dudt[i] = o3.y * T.DV + 0.5f * T.AVU;
-        W.vsig[i] = fmaxf(2.0f * o1.w, T.vmax);
-        if (MHD && mhd) {
-            const float divB = -T.DB * invrho;
-            const float divv = -T.DV * invrho;
-            float dbx = -(T.VBx - o4.x * T.DV) * invrho;
-            float dby = -(T.VBy - o4.y * T.DV) * invrho;
-            float dbz = -(T.VBz - o4.z * T.DV) * invrho;
-            float dpsi = 0.0f;
-            if (clean) {
-                dbx -= rho * T.PSx;
-                dby -= rho * T.PSy;
-                dbz -= rho * T.PSz;
-                const float psi = o4.w;
-                dpsi = -ch * ch * divB - psi * R.clean_sigma * ch * o2.y - 0.5f * psi * divv;
-            }
-            W.dbdt[0][i] = dbx;
-            W.dbdt[1][i] = dby;
-            W.dbdt[2][i] = dbz;
-            W.dpsidt[i] = dpsi;
-            W.divb[i] = divB;
-        }
-    }
+    if (k == 0 && tgt) force_write<MHD>(R, W, T, i, o1, o2, o3, o4, mhd, clean, borve, ch);
 }
 
 template <int CH, bool MIXED, bool MHD, int KT, int BAL, int RMAX>
