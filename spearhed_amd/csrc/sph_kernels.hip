@@ -55,6 +55,14 @@ __device__ __forceinline__ int wave_incl_scan(int v)
     return v;
 }
 
+// An LDS record at a byte offset from its array: the pair loops keep byte offsets so that a slot's
+// address is one 24-bit multiply-add (candidate bit x lane stride + the lane's first slot).
+template <class T>
+__device__ __forceinline__ const T& lds_at(const T* base, int byte_off)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
 __device__ __forceinline__ int wave_incl_max(int v)
 {
     int lane = threadIdx.x & 63;
@@ -674,10 +682,12 @@ __device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ 
                 if (keep) mw.put(msk, cbase + sb, lim);
                 cnt += (uint32_t)__popc(msk);
                 if (!BAL || K == 1) {
+                    const int kb = (int)sizeof(float4) * K;
+                    const int eb0 = (int)sizeof(float4) * (k + sb * K);  // byte offset of bit 0's slot
                     while (msk) {
                         const int ii = __builtin_ctz(msk);
                         msk &= msk - 1;
-                        const float4 p = sp[k + (int)__umul24((unsigned)(sb + ii), (unsigned)K)];
+                        const float4 p = lds_at(sp, (int)__umul24((unsigned)ii, (unsigned)kb) + eb0);
                         float dx, dy, dz;
                         const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
                         const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
@@ -748,7 +758,7 @@ __device__ __forceinline__ void force_pair(const float4& o0, const float4& o1, c
     // equal smoothing lengths (the records hold h, 1/h and the gradient factor derived from it)
     // give the same gradient bit for bit: the second evaluation is skipped then
     float s_j = s_i;
-    if (q2.x != o2.x) s_j = kgrad(kern, rr, invR, rr * q2.y, q2.z);
+    if (__builtin_expect(q2.x != o2.x, 0)) s_j = kgrad(kern, rr, invR, rr * q2.y, q2.z);
     const float dvx = o1.x - q1.x, dvy = o1.y - q1.y, dvz = o1.z - q1.z;
     const float vr = dvx * dx + dvy * dy + dvz * dz;
     const float m = q0.w;
@@ -760,7 +770,9 @@ __device__ __forceinline__ void force_pair(const float4& o0, const float4& o1, c
     A.DV += ms_i * vr;
     const float w = vr * invR;
     const float vs = o1.w + q1.w - beta * fminf(w, 0.0f);
-    A.vmax = fmaxf(A.vmax, vs);
+    // signal speeds are >= +0, so their float bits order as integers: an integer max is the float
+    // max without the NaN-quieting copy fmaxf needs for its loop-carried operand
+    A.vmax = __int_as_float(max(__float_as_int(A.vmax), __float_as_int(vs)));
     if (alpha > 0.0f) {  // uniform per run; receding pairs get mu = 0, hence Pi = 0
         const float hb = 0.5f * (o2.x + q2.x);
         const float mu = hb * fminf(vr, 0.0f) * fast_rcp(r2 + 0.01f * hb * hb);
@@ -973,15 +985,18 @@ __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ r
                 }
                 if (!tgt) msk = 0;
                 if (BAL == 0 || K == 1) {
+                    const int kb = (int)sizeof(float4) * K;
+                    const int eb0 = (int)sizeof(float4) * (k + sb * K);  // byte offset of bit 0's slot
                     while (msk) {
                         const int ii = __builtin_ctz(msk);
                         msk &= msk - 1;
-                        const int e = k + (int)__umul24((unsigned)(sb + ii), (unsigned)K);
+                        const int eb = (int)__umul24((unsigned)ii, (unsigned)kb) + eb0;
                         float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if constexpr (MHD) q4 = sp[4][e];
+                        if constexpr (MHD) q4 = lds_at(sp[4], eb);
                         // all of the pair's records read before any is used: one LDS wait per pair
                         // instead of one per record (force pass 3% faster; same arithmetic)
-                        const float4 q0 = sp[0][e], q1 = sp[1][e], q2 = sp[2][e], q3 = sp[3][e];
+                        const float4 q0 = lds_at(sp[0], eb), q1 = lds_at(sp[1], eb), q2 = lds_at(sp[2], eb),
+                                     q3 = lds_at(sp[3], eb);
                         __builtin_amdgcn_sched_barrier(0);
                         force_pair<MHD>(o0, o1, o2, o3, o4, q0, q1, q2, q3, q4, kern, alpha, beta, mhd, clean, A);
                     }
