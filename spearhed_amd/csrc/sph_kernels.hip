@@ -776,10 +776,11 @@ __device__ __forceinline__ void force_pair(const float4& o0, const float4& o1, c
     if (alpha > 0.0f) {  // uniform per run; receding pairs get mu = 0, hence Pi = 0
         const float hb = 0.5f * (o2.x + q2.x);
         const float mu = hb * fminf(vr, 0.0f) * fast_rcp(r2 + 0.01f * hb * hb);
-        const float cb = 0.5f * (o1.w + q1.w);
-        const float rb = 0.5f * (o2.w + q2.w);
-        const float Pi = (beta * mu - alpha * cb) * mu * fast_rcp(rb);
-        const float gv = Pi * 0.5f * (ms_i + ms_j);
+        // Pi = (beta mu - alpha cb) mu / rb, cb and rb the pair's mean sound speed and density, and
+        // gv = Pi (m s_i + m s_j) / 2. The halves are power-of-two scalings, exact in FP32: alpha cb
+        // = (alpha / 2)(c_i + c_j), and (1 / rb) / 2 = 1 / (rho_i + rho_j), so Ph = Pi / 2.
+        const float Ph = (beta * mu - (0.5f * alpha) * (o1.w + q1.w)) * mu * fast_rcp(o2.w + q2.w);
+        const float gv = Ph * (ms_i + ms_j);
         A.ax -= gv * dx;
         A.ay -= gv * dy;
         A.az -= gv * dz;
