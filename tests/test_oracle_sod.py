@@ -78,5 +78,6 @@ def test_full_size_sod_fixture_is_consistent():
     for k in S.FIELDS:
         assert l1[k] == pytest.approx(float(z[f"l1_{k}"]), rel=1e-12)
     assert 0.2 <= float(z["time"]) < 0.21 and int(z["steps"]) == 720
+    assert abs(float(z["time_plain"]) - float(z["time"])) < 5e-3 * float(z["time"])
     assert np.all(z["count"] > 0)
     np.testing.assert_allclose(z["centres"], S.CENTRES)
