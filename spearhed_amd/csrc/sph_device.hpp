@@ -156,6 +156,9 @@ __device__ __forceinline__ float kgrad(int kern, float rr, float invR, float q, 
 
 __device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// r from r^2 in one instruction (v_sqrt_f32, about 1 ulp): sqrt(0) = 0, so coincident particles
+// need no guard, where r2 * rsq(r2) does (0 * inf)
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Squared distance from the own pair coordinate to a staged (pre-shifted) neighbour, as
 // fma(dz, dz, fma(dy, dy, dx*dx)) with every other contraction off: the cull decision and the

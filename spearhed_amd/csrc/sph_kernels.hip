@@ -739,8 +739,7 @@ __device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ 
                         const float4 p = lds_at(sp, (int)__umul24((unsigned)ii, (unsigned)kb) + eb0);
                         float dx, dy, dz;
                         const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
-                        const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
-                        acc += p.w * kshape(kern, r2 * invR * invh);
+                        acc += p.w * kshape(kern, fast_sqrt(r2) * invh);
                     }
                 } else {
                     const int gtot = compact_group(msk, sb, lm, lst);
@@ -749,8 +748,7 @@ __device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ 
                         const float4 p = sp[lst[lm.gfirst * 32 + q]];
                         float dx, dy, dz;
                         const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
-                        const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
-                        acc += p.w * kshape(kern, r2 * invR * invh);
+                        acc += p.w * kshape(kern, fast_sqrt(r2) * invh);
                     }
                     __syncthreads();
                 }
@@ -1211,8 +1209,7 @@ __global__ __launch_bounds__(64) void k_density_cell(const RunDev* __restrict__ 
                     const float4 p = sp[cc - base + ii];
                     float dx, dy, dz;
                     const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
-                    const float invR = r2 > 0.0f ? fast_rsq(r2) : 0.0f;
-                    acc += p.w * kshape(kern, r2 * invR * invh);
+                    acc += p.w * kshape(kern, fast_sqrt(r2) * invh);
                 }
             }
             __syncthreads();
