@@ -220,6 +220,7 @@ def run_serving(args):
             "admission_and_upload_ms_total": summary["admission_ms_total"],
             "retire_ms_total": summary["retire_ms_total"], "move_ms_total": summary["move_ms_total"],
             "fill_ms_total": summary["fill_ms_total"], "harvest_ms_total": summary["harvest_ms_total"],
+            "idle_wait_ms_total": summary["idle_wait_ms_total"],
             "admission_ms_p50": summary["admission_ms_p50"], "steps": summary["steps"], "wall_s": wall,
             "device_busy_s": summary["steps"] and sum(srv.stats.step_ms) * 1e-3}
 

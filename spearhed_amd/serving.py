@@ -126,6 +126,7 @@ class ServeStats:
     fill_ms: List[float] = field(default_factory=list)     # uploads / device fills of admitted runs
     step_wall_ms: List[float] = field(default_factory=list)  # host-side interval between step completions
     harvest_ms: List[float] = field(default_factory=list)    # responses turned into results
+    idle_ms: List[float] = field(default_factory=list)       # no run resident: waiting for an arrival
 
     def summary(self) -> dict:
         s = np.array(self.step_ms) if self.step_ms else np.zeros(1)
@@ -145,6 +146,7 @@ class ServeStats:
             "fill_ms_total": float(np.sum(self.fill_ms)) if self.fill_ms else 0.0,
             "harvest_ms_total": float(np.sum(self.harvest_ms)) if self.harvest_ms else 0.0,
             "admission_ms_p50": float(np.percentile(self.admit_ms, 50)) if self.admit_ms else 0.0,
+            "idle_wait_ms_total": float(np.sum(self.idle_ms)) if self.idle_ms else 0.0,
         }
 
 
@@ -344,7 +346,9 @@ class Server:
                 self._harvest()
                 last_done = None
                 if self.queue:  # idle until the next arrival
+                    t_idle = time.perf_counter()
                     time.sleep(max(0.0, min(self.queue[0][0] - self.now(), 0.01)))
+                    self.stats.idle_ms.append((time.perf_counter() - t_idle) * 1e3)
         if inflight is not None:  # stopped by max_steps with a step in flight
             self._pending = inflight[1] + self._pending
         self._retire_and_admit(float("-inf"))  # retire the last runs, admit nothing
