@@ -179,7 +179,8 @@ int sph_sync(sph_ctx* ctx);
 /* Marks on the context's stream, for hosts that keep work in flight (a serving loop that enqueues
  * the next step before the last one has finished): sph_mark records the current point of the stream
  * and returns its ticket (1, 2, ...); sph_mark_wait blocks until the stream has passed the point of
- * that ticket (every earlier point too). Waiting on a ticket more than 64 marks old waits for a
+ * that ticket (every earlier point too), and until the copy-outs (sph_download_async) enqueued before
+ * the mark have landed. Waiting on a ticket more than 64 marks old waits for a
  * later point. The analogue of waiting on one step's sender rather than the whole queue. */
 int sph_mark(sph_ctx* ctx, uint64_t* ticket);
 int sph_mark_wait(sph_ctx* ctx, uint64_t ticket);
@@ -245,10 +246,14 @@ int sph_download(sph_ctx* ctx, int32_t run, sph_host_soa* dst);
 /* Copy-out for responses (the step after the path: the reference copies selected species fields
  * out for output, spmacc/particles/algorithms/CopyParticlesToDynSoA.hpp, include/spearhed/memory.hpp:42-49):
  * enqueue device-to-host copies of the non-NULL columns of `dst` (and, if `cons` is non-NULL, the
- * run's FP64 sums and next dt as its last step left them) on the context's stream and return. The
- * host buffers are valid after the next call that synchronises the stream (sph_sync, sph_download,
- * sph_upload); they should be pinned (sph_host_alloc) for the copies to run asynchronously.
- * Positions are available in mixed-precision contexts only. */
+ * run's FP64 sums and next dt as its last step left them) and return. The copies hold the run as it
+ * is at this point of the context's stream. In a pool (sph_pool_create) whose snapshot ring (16
+ * bytes per particle of capacity, counted in sph_pool_bytes) holds the columns, they are first
+ * copied within the device on the context's stream and then go out to the host on a second stream
+ * of the library's, overlapping the steps enqueued after them; otherwise they go out on the
+ * context's stream. The host buffers are valid after sph_sync, sph_download, sph_upload, or
+ * sph_mark_wait on a mark taken after this call; they should be pinned (sph_host_alloc) for the
+ * copies to run asynchronously. Positions are available in mixed-precision contexts only. */
 int sph_download_async(sph_ctx* ctx, int32_t run, const sph_host_soa* dst, sph_conserved* cons);
 /* Pinned host memory for sph_download_async / sph_upload staging. */
 int sph_host_alloc(int64_t bytes, void** out);

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,22 @@ struct sph_ctx {
     static constexpr int kMarks = 64;
     hipEvent_t marks[kMarks] = {};
     uint64_t next_mark = 1;
+    // overlapped copy-out (pools): sph_download_async snapshots a run's columns into a device ring
+    // on the context's stream, and a second stream copies them to the host while the next steps
+    // run (the next step's re-sort rewrites the state the run was downloaded from)
+    struct SnapCopy {
+        int64_t off, bytes;
+        hipEvent_t done;  // recorded on copy_stream after the copies out of [off, off + bytes)
+    };
+    char* snap = nullptr;
+    int64_t snap_bytes = 0;
+    int64_t snap_head = 0;  // end of the newest region in use
+    std::deque<SnapCopy> snap_q;  // regions whose copies out may be in flight, oldest first
+    std::vector<hipEvent_t> snap_free;
+    hipEvent_t snap_ready = nullptr;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_marks[kMarks] = {};
+    bool copy_marked[kMarks] = {};
     bool binned = false;   // keys + histogram valid for the current positions
     bool sorted = false;   // buffers sorted by (cell, id) for the current positions
     bool dens_valid = false;
@@ -235,7 +252,12 @@ void carve(Carver& cv, sph_ctx* c, int64_t N, int ncell_cap, int nkb, int nruns)
     W.move_tab = cv.take<int64_t>((int64_t)nruns * 3);
     W.run_src = cv.take<int32_t>(nruns);
     W.run_tmp = cv.take<double>((int64_t)nruns * 3);
+    c->snap = c->snap_bytes > 0 ? cv.take<char>(c->snap_bytes) : nullptr;
 }
+
+// Device ring of a pool for snapshots of downloaded runs: 16 bytes per particle of capacity, so a
+// run of up to a quarter of the capacity with a 64-byte response goes out through it.
+int64_t pool_snap_bytes(int64_t max_particles) { return ((max_particles * 16 + 255) / 256) * 256; }
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
@@ -545,6 +567,53 @@ int fetch_col(sph_ctx* c, T* dst_host, const T* src_dev, int64_t n)
     return SPH_OK;
 }
 
+// Wait for every copy out of the snapshot ring (host-side) and empty the ring.
+int join_copies(sph_ctx* c)
+{
+    if (!c->copy_stream) return SPH_OK;
+    HIPTRY(c, hipStreamSynchronize(c->copy_stream));
+    for (const auto& e : c->snap_q) c->snap_free.push_back(e.done);
+    c->snap_q.clear();
+    c->snap_head = 0;
+    return SPH_OK;
+}
+
+// A region of `need` bytes in the snapshot ring, at `off`. Regions whose copies out have finished
+// are taken back first; if that is not enough, the oldest regions are taken back with the context's
+// stream waiting (on the device) for their copies out. Returns false if the ring is too small.
+bool snap_alloc(sph_ctx* c, int64_t need, int64_t& off, int& rc)
+{
+    rc = SPH_OK;
+    if (need > c->snap_bytes) return false;
+    while (!c->snap_q.empty() && hipEventQuery(c->snap_q.front().done) == hipSuccess) {
+        c->snap_free.push_back(c->snap_q.front().done);
+        c->snap_q.pop_front();
+    }
+    for (;;) {
+        if (c->snap_q.empty()) {
+            off = 0;
+            break;
+        }
+        const int64_t tail = c->snap_q.front().off;  // start of the oldest region in use
+        if (c->snap_head > tail) {                     // in use: [tail, head)
+            if (c->snap_bytes - c->snap_head >= need) { off = c->snap_head; break; }
+            if (tail >= need) { off = 0; break; }
+        } else if (tail - c->snap_head >= need) {      // in use: [tail, end) and [0, head)
+            off = c->snap_head;
+            break;
+        }
+        const hipError_t e = hipStreamWaitEvent(c->stream, c->snap_q.front().done, 0);
+        if (e != hipSuccess) {
+            rc = fail(c, SPH_E_DEVICE, std::string("hipStreamWaitEvent: ") + hipGetErrorString(e));
+            return false;
+        }
+        c->snap_free.push_back(c->snap_q.front().done);
+        c->snap_q.pop_front();
+    }
+    c->snap_head = off + need;
+    return true;
+}
+
 // Supercells a run may need: its own grid or the finest (cell_factor 1) the autotuner may pick.
 int64_t cells_needed(const sph_run_config& cfg, std::string& why)
 {
@@ -634,6 +703,7 @@ int sph_pool_create(int64_t max_particles, int64_t max_cells, int32_t max_runs, 
     c->has_mhd = mhd != 0;
     const int64_t cap_kb = max_particles / 256 + max_runs;
     if (cap_kb >= (1LL << 31)) return fail(c, SPH_E_CAPACITY, "too many particle blocks");
+    c->snap_bytes = pool_snap_bytes(max_particles);
     int rc = open_stream(c, stream);
     if (!rc) rc = reserve(c, max_particles, max_cells, max_runs, (int)cap_kb);
     return rc;
@@ -707,7 +777,14 @@ int sph_ctx_destroy(sph_ctx* c)
 {
     if (!c) return SPH_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->copy_stream) hipStreamSynchronize(c->copy_stream);
     invalidate_graph(c);
+    for (const auto& e : c->snap_q) hipEventDestroy(e.done);
+    for (auto e : c->snap_free) hipEventDestroy(e);
+    for (auto e : c->copy_marks)
+        if (e) hipEventDestroy(e);
+    if (c->snap_ready) hipEventDestroy(c->snap_ready);
+    if (c->copy_stream) hipStreamDestroy(c->copy_stream);
     for (auto e : c->prof_ev) hipEventDestroy(e);
     for (auto e : c->stage_ev)
         if (e) hipEventDestroy(e);
@@ -727,9 +804,17 @@ int sph_mark(sph_ctx* c, uint64_t* ticket)
 {
     if (!c || !ticket) return SPH_E_INVALID;
     const uint64_t t = c->next_mark;
-    hipEvent_t& e = c->marks[t % sph_ctx::kMarks];
+    const int slot = (int)(t % sph_ctx::kMarks);
+    hipEvent_t& e = c->marks[slot];
     if (!e) HIPTRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPTRY(c, hipEventRecord(e, c->stream));
+    // and the point of the copy-out stream: copies out enqueued before the mark have landed too
+    c->copy_marked[slot] = c->copy_stream != nullptr;
+    if (c->copy_stream) {
+        hipEvent_t& ce = c->copy_marks[slot];
+        if (!ce) HIPTRY(c, hipEventCreateWithFlags(&ce, hipEventDisableTiming));
+        HIPTRY(c, hipEventRecord(ce, c->copy_stream));
+    }
     ++c->next_mark;
     *ticket = t;
     return SPH_OK;
@@ -739,7 +824,9 @@ int sph_mark_wait(sph_ctx* c, uint64_t ticket)
 {
     if (!c || ticket == 0 || ticket >= c->next_mark) return SPH_E_INVALID;
     // a slot reused by a later mark waits for that later point: the stream has then passed both
-    HIPTRY(c, hipEventSynchronize(c->marks[ticket % sph_ctx::kMarks]));
+    const int slot = (int)(ticket % sph_ctx::kMarks);
+    HIPTRY(c, hipEventSynchronize(c->marks[slot]));
+    if (c->copy_marked[slot]) HIPTRY(c, hipEventSynchronize(c->copy_marks[slot]));
     return SPH_OK;
 }
 
@@ -747,7 +834,7 @@ int sph_sync(sph_ctx* c)
 {
     if (!c) return SPH_E_INVALID;
     HIPTRY(c, hipStreamSynchronize(c->stream));
-    return SPH_OK;
+    return join_copies(c);
 }
 
 int sph_get_grid(sph_ctx* c, int32_t run, sph_grid_info* o)
@@ -815,7 +902,8 @@ int64_t sph_bytes_per_run(const sph_run_config* cfg, int64_t n, int32_t precisio
     const int64_t pos = precision == SPH_PREC_MIXED ? 24 : 12;
     const int64_t mhd = cfg->mhd ? 1 : 0;
     const int64_t state = 8 + 2 + pos + 12 + 16 + mhd * 16;
-    const int64_t work = 12 + 4 + 4 + 4 + 12 + 8 + 64 + mhd * (12 + 4 + 4 + 16);
+    // work arrays, and the pool's copy-out snapshot ring (16 B per particle of capacity)
+    const int64_t work = 12 + 4 + 4 + 4 + 12 + 8 + 64 + mhd * (12 + 4 + 4 + 16) + 16;
     const int64_t blocks = (n + 255) / 256;  // FP64 partial sums (6 doubles) + block->run entry per block
     // dt, time, cleaning speed, sums, flag, move entry, source index, saved scalars
     const int64_t per_run = 8 + 8 + 4 + 8 * 8 + 4 + 3 * 8 + 4 + 3 * 8;
@@ -829,6 +917,7 @@ int64_t sph_pool_bytes(int64_t max_particles, int64_t max_cells, int32_t max_run
     tmp.prec = precision;
     tmp.mixed = precision == SPH_PREC_MIXED;
     tmp.has_mhd = mhd != 0;
+    tmp.snap_bytes = pool_snap_bytes(max_particles);
     Carver dry{nullptr, 0, true};
     carve(dry, &tmp, max_particles, (int)max_cells, (int)(max_particles / 256 + max_runs), max_runs);
     return (int64_t)dry.off + 256 + (int64_t)sizeof(RunDev) * max_runs;
@@ -907,7 +996,7 @@ int sph_upload(sph_ctx* c, int32_t run, const sph_host_soa* s)
     c->binned = false;
     c->sorted = false;
     c->dens_valid = false;
-    return SPH_OK;
+    return join_copies(c);  // earlier asynchronous copy-outs have landed too
 }
 
 int64_t sph_lattice_sites(const sph_lattice_block* blocks, int32_t n_blocks)
@@ -964,38 +1053,85 @@ int sph_download_async(sph_ctx* c, int32_t run, const sph_host_soa* d, sph_conse
         return fail(c, SPH_E_INVALID, "download_async: positions need conversion in FP32 contexts; use sph_download");
     const int64_t n = R.np, o = R.p0;
     const StateBuf& S = c->S[c->cur];
-    int rc = 0;
-    rc |= fetch_col(c, d->id, S.id + o, n);
-    rc |= fetch_col(c, d->mask, S.mask + o, n);
-    rc |= fetch_col(c, d->species, S.species + o, n);
+    struct Col {
+        void* dst;
+        const void* src;
+        int64_t bytes;
+    };
+    std::vector<Col> cols;
+    auto col = [&](void* dst, const void* src, size_t elem) {
+        if (dst && src) cols.push_back({dst, src, (int64_t)elem * n});
+    };
+    col(d->id, S.id + o, 8);
+    col(d->mask, S.mask + o, 1);
+    col(d->species, S.species + o, 1);
     for (int a = 0; a < 3; ++a) {
-        if (c->mixed) rc |= fetch_col(c, d->pos[a], S.pos64[a] + o, n);
-        rc |= fetch_col(c, d->vel[a], S.vel[a] + o, n);
-        rc |= fetch_col(c, d->dvdt[a], c->W.dvdt[a] + o, n);
+        if (c->mixed) col(d->pos[a], S.pos64[a] + o, 8);
+        col(d->vel[a], S.vel[a] + o, 4);
+        col(d->dvdt[a], c->W.dvdt[a] + o, 4);
         if (c->has_mhd) {
-            rc |= fetch_col(c, d->bfield[a], S.B[a] + o, n);
-            rc |= fetch_col(c, d->dbdt[a], c->W.dbdt[a] + o, n);
+            col(d->bfield[a], S.B[a] + o, 4);
+            col(d->dbdt[a], c->W.dbdt[a] + o, 4);
         }
     }
-    rc |= fetch_col(c, d->mass, S.mass + o, n);
-    rc |= fetch_col(c, d->density, S.rho + o, n);
-    rc |= fetch_col(c, d->smoothing_length, S.h + o, n);
-    rc |= fetch_col(c, d->internal_energy, S.u + o, n);
-    rc |= fetch_col(c, d->dudt, c->W.dudt + o, n);
-    rc |= fetch_col(c, d->vsig, c->W.vsig + o, n);
-    rc |= fetch_col(c, d->n_neighbours, c->W.nnb + o, n);
+    col(d->mass, S.mass + o, 4);
+    col(d->density, S.rho + o, 4);
+    col(d->smoothing_length, S.h + o, 4);
+    col(d->internal_energy, S.u + o, 4);
+    col(d->dudt, c->W.dudt + o, 4);
+    col(d->vsig, c->W.vsig + o, 4);
+    col(d->n_neighbours, c->W.nnb + o, 4);
     if (c->has_mhd) {
-        rc |= fetch_col(c, d->psi, S.psi + o, n);
-        rc |= fetch_col(c, d->dpsidt, c->W.dpsidt + o, n);
-        rc |= fetch_col(c, d->divb, c->W.divb + o, n);
+        col(d->psi, S.psi + o, 4);
+        col(d->dpsidt, c->W.dpsidt + o, 4);
+        col(d->divb, c->W.divb + o, 4);
     }
-    if (cons) {
-        // the FP64 sums and CFL dt that the run's last step (or reduction) left on the device
-        static_assert(sizeof(sph_conserved) == 8 * sizeof(double), "sph_conserved is 8 doubles");
-        HIPTRY(c, hipMemcpyAsync(cons, c->W.cons + (int64_t)run * 8, sizeof(sph_conserved), hipMemcpyDeviceToHost,
-                                 c->stream));
+    // the FP64 sums and CFL dt that the run's last step (or reduction) left on the device
+    static_assert(sizeof(sph_conserved) == 8 * sizeof(double), "sph_conserved is 8 doubles");
+    if (cons) cols.push_back({cons, c->W.cons + (int64_t)run * 8, (int64_t)sizeof(sph_conserved)});
+
+    // Pools snapshot the columns into their device ring on the context's stream (device-to-device,
+    // at HBM speed) and copy the snapshot to the host on a second stream, so the copies over PCIe
+    // overlap the steps that follow; otherwise, or when the ring is too small for the run, or while
+    // the stream is being captured into a graph, the copies go out on the context's stream.
+    int64_t need = 0;
+    for (const Col& k : cols) need += (k.bytes + 255) & ~int64_t(255);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIPTRY(c, hipStreamIsCapturing(c->stream, &cap));
+    int64_t off = 0;
+    int rc = SPH_OK;
+    if (c->snap && cap == hipStreamCaptureStatusNone && need > 0 && snap_alloc(c, need, off, rc)) {
+        if (!c->copy_stream) {
+            HIPTRY(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+            HIPTRY(c, hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
+        }
+        int64_t at = off;
+        for (const Col& k : cols) {
+            HIPTRY(c, hipMemcpyAsync(c->snap + at, k.src, (size_t)k.bytes, hipMemcpyDeviceToDevice, c->stream));
+            at += (k.bytes + 255) & ~int64_t(255);
+        }
+        HIPTRY(c, hipEventRecord(c->snap_ready, c->stream));
+        HIPTRY(c, hipStreamWaitEvent(c->copy_stream, c->snap_ready, 0));
+        at = off;
+        for (const Col& k : cols) {
+            HIPTRY(c, hipMemcpyAsync(k.dst, c->snap + at, (size_t)k.bytes, hipMemcpyDeviceToHost, c->copy_stream));
+            at += (k.bytes + 255) & ~int64_t(255);
+        }
+        hipEvent_t done = nullptr;
+        if (!c->snap_free.empty()) {
+            done = c->snap_free.back();
+            c->snap_free.pop_back();
+        } else {
+            HIPTRY(c, hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        }
+        HIPTRY(c, hipEventRecord(done, c->copy_stream));
+        c->snap_q.push_back({off, need, done});
+        return SPH_OK;
     }
-    return rc ? SPH_E_DEVICE : SPH_OK;
+    if (rc) return rc;
+    for (const Col& k : cols)
+        HIPTRY(c, hipMemcpyAsync(k.dst, k.src, (size_t)k.bytes, hipMemcpyDeviceToHost, c->stream));
+    return SPH_OK;
 }
 
 int sph_download(sph_ctx* c, int32_t run, sph_host_soa* d)
@@ -1041,7 +1177,7 @@ int sph_download(sph_ctx* c, int32_t run, sph_host_soa* d)
         for (int a = 0; a < 3; ++a)
             if (d->pos[a])
                 for (int64_t i = 0; i < n; ++i) d->pos[a][i] = (double)tmp[a][i];
-    return SPH_OK;
+    return join_copies(c);  // earlier asynchronous copy-outs have landed too
 }
 
 int sph_stage_drift(sph_ctx* c, double dt)

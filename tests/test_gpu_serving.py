@@ -211,6 +211,56 @@ def test_served_response_fields_equal_alone():
             assert got.conserved[k] == ca[k], k
 
 
+def test_pool_copy_out_holds_the_run_at_the_call():
+    """In a pool, sph_download_async snapshots the run into the pool's device ring on the pool's
+    stream and copies the snapshot out on a second stream while later steps run. Each copy holds the
+    run as it was where the call was enqueued, though the steps after it rewrite the run's state;
+    consecutive copy-outs take the ring's space back (the pool's stream waits for the older copies
+    out); sph_mark_wait on a later mark sees every copy landed."""
+    from spearhed_amd.stepper import PinnedRing, download_async
+    runs = [ic.kelvin_helmholtz(nt, 700 + r) for r, nt in enumerate([3000, 5000])]
+    fields = ("id", "pos", "vel", "rho", "u", "B", "psi", "nnb")
+    # the ring holds 16 B per particle of capacity (384 kB here): one response of these fields for
+    # the 5000-particle run (~363 kB) fits, two responses in a row do not
+    cap = 24000
+    print("pool copy-out: runs", [p.n for _, p in runs], flush=True)
+    rings = [PinnedRing(1 << 20) for _ in range(3)]
+    taken = []  # (run, steps taken when copied, arrays, conserved)
+    try:
+        with Pool(cap, 4000, 4, "mixed", True) as pool:
+            pool.set_runs([], [(c, p.n) for c, p in runs])
+            for r, (_, p) in enumerate(runs):
+                pool.upload(r, p)
+            pool.prime()
+            pool.step(2)
+            plan = [(1, 2), (0, 3), (1, 4)]  # run copied out after that many steps
+            for k, (run, steps) in enumerate(plan):
+                arrays, cons = download_async(pool, run, fields, rings[k])
+                taken.append((run, steps, arrays, cons))
+                pool.step(1)  # rewrites the state the copy was taken from
+            mark = pool.mark()
+            pool.step(2)
+            pool.wait_mark(mark)
+            snap = [(run, steps, {f: np.array(a) for f, a in arrays.items()}, cons.as_dict())
+                    for run, steps, arrays, cons in taken]
+            pool.sync()
+            final = [pool.download(r).sorted_by_id() for r in range(2)]
+    finally:
+        for ring in rings:
+            ring.close()
+    for run, steps, got, cons in snap:
+        cfg, ps = runs[run]
+        a, ca = _alone(cfg, ps, steps)
+        order = np.argsort(got["id"])
+        assert np.array_equal(got["id"][order], a.id)
+        for f in ("pos", "vel", "rho", "u", "B", "psi", "nnb"):
+            assert np.array_equal(got[f][..., order], getattr(a, f)), (run, steps, f)
+        for k in ("time", "dt_next", "energy"):
+            assert cons[k] == ca[k], (run, steps, k)
+    for r, (cfg, ps) in enumerate(runs):
+        _same(_alone(cfg, ps, 7)[0], final[r])
+
+
 def test_set_runs_rejects_block_overflow_and_leaves_runs_intact():
     """A context from sph_ctx_create has exactly the particle-kernel blocks of its runs: two runs of
     256 particles hold 2 blocks, so a new set of 1 + 511 particles (3 blocks) fits the particle count
