@@ -1,5 +1,5 @@
 """bench.py's host-side contract (CPU only): the one-GPU guard, the two CPU baselines and the
-measured-traffic lookup. The GPU legs run on the GPU box (`python bench.py`)."""
+measured-traffic lookup. The GPU legs run on an MI355X (`python bench.py`)."""
 from __future__ import annotations
 
 import json
