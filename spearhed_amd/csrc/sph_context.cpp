@@ -27,6 +27,9 @@ struct sph_ctx {
     int nruns = 0;
     std::vector<sph_run_config> cfg;
     std::vector<RunDev> runs_h;
+    // per run: every particle has the same smoothing length (set by sph_upload and sph_init_run;
+    // the state never changes h), so the force pass may take its one-h instantiation
+    std::vector<uint8_t> one_h;
     RunDev* runs_d = nullptr;
     int64_t ntot = 0;
     int ncell_tot = 0;
@@ -282,6 +285,8 @@ LaunchCtx lctx(sph_ctx* c)
     L.S = c->S[c->cur];
     L.force_chunk = c->tune.force_chunk;
     L.balance = c->tune.pair_balance < 0 ? 0 : (c->tune.pair_balance > 2 ? 2 : c->tune.pair_balance);
+    L.one_h = c->nruns > 0 ? 1 : 0;
+    for (int r = 0; r < c->nruns; ++r) L.one_h = L.one_h && c->one_h[r];
     L.kern = c->nruns > 0 ? c->cfg[0].kernel : -1;
     for (int r = 1; r < c->nruns; ++r)
         if (c->cfg[r].kernel != L.kern) L.kern = -1;
@@ -498,6 +503,10 @@ int layout(sph_ctx* c, const std::vector<sph_run_config>& cfg, const std::vector
     if (c0 > c->ncell_cap) return fail(c, SPH_E_CAPACITY, "supercell capacity exceeded");
     if (kb > c->cap_kb) return fail(c, SPH_E_CAPACITY, "block table capacity exceeded");
     const int n_old = c->nruns;
+    std::vector<uint8_t> one_h(nr, 0);  // kept runs keep their flag; new runs have no data yet
+    for (int r = 0; r < nr; ++r)
+        if (src[r] >= 0 && src[r] < (int)c->one_h.size()) one_h[r] = c->one_h[src[r]];
+    c->one_h = one_h;
     c->nruns = nr;
     c->cfg = cfg;
     c->runs_h = runs;
@@ -960,9 +969,13 @@ int sph_upload(sph_ctx* c, int32_t run, const sph_host_soa* s)
         }
         if (dup) return fail(c, SPH_E_INVALID, "upload: particle ids must be unique within a run");
     }
-    for (int64_t i = 0; i < s->n; ++i)
+    bool one_h = true;
+    for (int64_t i = 0; i < s->n; ++i) {
         if (!(s->smoothing_length[i] > 0.0f) || !std::isfinite(s->smoothing_length[i]))
             return fail(c, SPH_E_INVALID, "upload: smoothing_length must be finite and > 0");
+        one_h = one_h && s->smoothing_length[i] == s->smoothing_length[0];
+    }
+    c->one_h[run] = one_h ? 1 : 0;
     HIPTRY(c, hipStreamSynchronize(c->stream));
     const int64_t n = s->n, o = R.p0;
     StateBuf& S = c->S[c->cur];
@@ -1023,6 +1036,7 @@ int sph_init_run(sph_ctx* c, int32_t run, const sph_lattice_block* blocks, int32
     HIPTRY(c, launch_init_lattice(blocks, n_blocks, R.np, R.p0, seed, jitter, cfg.h0, cfg.gamma, c->mixed ? 1 : 0,
                                   c->has_mhd ? 1 : 0, c->S[c->cur], c->stream));
     // as an upload: time restarts at 0 and the next step primes the run (enqueued, nothing waits)
+    c->one_h[run] = 1;  // the fill gives every particle the run's h0
     HIPTRY(c, launch_restart_run(c->runs_d, c->W, run, c->stream));
     if (c->binned) HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_tot, c->stream));
     c->binned = false;

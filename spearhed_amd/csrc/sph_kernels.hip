@@ -791,7 +791,9 @@ struct ForceAcc {
     float Ti, Tj, TBx, TBy, TBz, VBx, VBy, VBz, DB, PSx, PSy, PSz;
 };
 
-template <bool MHD>
+// ONE_H: every particle of the pair's run has the same smoothing length h; the pair then takes
+// s_j = s_i and the mean h = h without testing or forming them (the same bits: 0.5 (h + h) = h).
+template <bool MHD, bool ONE_H = false>
 __device__ __forceinline__ void force_pair(const float4& o0, const float4& o1, const float4& o2, const float4& o3,
                                            const float4& o4, const float4& q0, const float4& q1, const float4& q2,
                                            const float4& q3, const float4& q4, int kern, float alpha, float beta,
@@ -805,7 +807,7 @@ __device__ __forceinline__ void force_pair(const float4& o0, const float4& o1, c
     // equal smoothing lengths (the records hold h, 1/h and the gradient factor derived from it)
     // give the same gradient bit for bit: the second evaluation is skipped then
     float s_j = s_i;
-    if (__builtin_expect(q2.x != o2.x, 0)) s_j = kgrad(kern, rr, invR, rr * q2.y, q2.z);
+    if (!ONE_H && __builtin_expect(q2.x != o2.x, 0)) s_j = kgrad(kern, rr, invR, rr * q2.y, q2.z);
     const float dvx = o1.x - q1.x, dvy = o1.y - q1.y, dvz = o1.z - q1.z;
     const float vr = dvx * dx + dvy * dy + dvz * dz;
     const float m = q0.w;
@@ -821,7 +823,7 @@ __device__ __forceinline__ void force_pair(const float4& o0, const float4& o1, c
     // max without the NaN-quieting copy fmaxf needs for its loop-carried operand
     A.vmax = __int_as_float(max(__float_as_int(A.vmax), __float_as_int(vs)));
     if (alpha > 0.0f) {  // uniform per run; receding pairs get mu = 0, hence Pi = 0
-        const float hb = 0.5f * (o2.x + q2.x);
+        const float hb = ONE_H ? o2.x : 0.5f * (o2.x + q2.x);
         const float mu = hb * fminf(vr, 0.0f) * fast_rcp(r2 + 0.01f * hb * hb);
         // Pi = (beta mu - alpha cb) mu / rb, cb and rb the pair's mean sound speed and density, and
         // gv = Pi (m s_i + m s_j) / 2. The halves are power-of-two scalings, exact in FP32: alpha cb
@@ -955,7 +957,7 @@ __device__ __forceinline__ void force_finish(const RunDev& R, WorkBuf& W, const 
     if (k == 0 && tgt) force_write<MHD>(R, W, T, i, o1, o2, o3, o4, mhd, clean, borve, ch);
 }
 
-template <int CH, bool MIXED, bool MHD, int KT, int BAL, int RMAX>
+template <int CH, bool MIXED, bool MHD, int KT, int BAL, int RMAX, bool ONE_H = false>
 __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ runs, int nruns, int ncell_tot,
                                             int nblk_total, StateBuf& S, WorkBuf& W, int kmax, NbListR<RMAX>& L,
                                             float4 (&sp)[MHD ? 5 : 4][CH], ListT<CH> (&lst)[BAL == 1 ? 64 * 32 : 1])
@@ -1048,7 +1050,8 @@ __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ r
                         const float4 q0 = lds_at(sp[0], eb), q1 = lds_at(sp[1], eb), q2 = lds_at(sp[2], eb),
                                      q3 = lds_at(sp[3], eb);
                         __builtin_amdgcn_sched_barrier(0);
-                        force_pair<MHD>(o0, o1, o2, o3, o4, q0, q1, q2, q3, q4, kern, alpha, beta, mhd, clean, A);
+                        force_pair<MHD, ONE_H>(o0, o1, o2, o3, o4, q0, q1, q2, q3, q4, kern, alpha, beta, mhd, clean,
+                                               A);
                     }
                 } else {
                     const int gtot = compact_group(msk, sb, lm, lst);
@@ -1069,7 +1072,7 @@ __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ r
     }
 }
 
-template <int CH, bool MIXED, bool MHD, int KT, int BAL, int RMAX = 1, bool GRID = false>
+template <int CH, bool MIXED, bool MHD, int KT, int BAL, int RMAX = 1, bool GRID = false, bool ONE_H = false>
 __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
                                                int nblk_total, StateBuf S, WorkBuf W, int kmax, int balance)
 {
@@ -1085,8 +1088,8 @@ __global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, 
                                                        lst);
     } else {
         // one supercell per wave
-        forces_wave<CH, MIXED, MHD, KT, BAL, RMAX>((int)blockIdx.x, runs, nruns, ncell_tot, nblk_total, S, W, kmax,
-                                                   L, sp, lst);
+        forces_wave<CH, MIXED, MHD, KT, BAL, RMAX, ONE_H>((int)blockIdx.x, runs, nruns, ncell_tot, nblk_total, S, W,
+                                                          kmax, L, sp, lst);
     }
 }
 
@@ -1741,6 +1744,9 @@ static void forces_ch(const LaunchCtx& c, int nblk)
     if (grid < nblk)
         SPH_PAIR_DISPATCH(k_forces, grid, CH, SPH_COMMA BAL SPH_COMMA 1 SPH_COMMA true, c.runs, c.nruns, c.ncell_tot,
                           nblk, c.S, c.W, c.kmax, c.balance);
+    else if (BAL == 0 && c.one_h)  // one smoothing length per run: the pair loop neither tests nor averages h
+        SPH_PAIR_DISPATCH(k_forces, nblk, CH, SPH_COMMA BAL SPH_COMMA 1 SPH_COMMA false SPH_COMMA true, c.runs, c.nruns,
+                          c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
     else
         SPH_PAIR_DISPATCH(k_forces, nblk, CH, SPH_COMMA BAL, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax,
                           c.balance);

@@ -19,6 +19,7 @@ struct LaunchCtx {
     int kmax;     // max lanes per target in the pair passes
     int balance;  // share a target's accepted pairs among its K lanes
     int kern;     // smoothing kernel of every run (SPH_KERNEL_*), or -1 when runs differ
+    int one_h;    // every run's particles share one smoothing length (the run's own)
     int rmax;     // largest stencil reach of the runs: 1 (supercells of edge >= H) or 2 (edge >= H/2; pair
                   // passes take the chunked lane loop, pair_balance 0)
     int persistent_waves;  // pair passes: 0 = one wave per supercell; else a persistent grid of this many

@@ -324,6 +324,32 @@ def test_mixed_kernel_batch():
             assert np.array_equal(getattr(g, f), getattr(alone, f)), (cfg.kernel, f)
 
 
+def test_one_h_instantiation_equals_general():
+    """A run whose particles share one smoothing length takes the force pass's one-h instantiation
+    (no test of h_j against h_i, the mean h taken as h). In a context that also holds a run of two
+    smoothing lengths the same run takes the general instantiation: its results are bit-identical
+    either way, and the two-h run (every other particle at 0.9 h, so pairs of unequal h take the
+    second kernel gradient and the mean h) meets the oracle contract."""
+    cfg, ps = _turb(12, 5)
+    tcfg, tps = _turb(10, 6)
+    tps.h[1::2] = np.float32(0.9) * tps.h[1::2]
+    assert np.unique(ps.h).size == 1 and np.unique(tps.h).size == 2
+    print(f"one-h vs general: n={ps.n}, two-h run n={tps.n}", flush=True)
+    alone = gpu_stages(cfg, ps, "mixed")
+    with Context([cfg, tcfg], [ps.n, tps.n], precision="mixed") as ctx:
+        ctx.upload(0, ps)
+        ctx.upload(1, tps)
+        ctx.resort()
+        ctx.density()
+        ctx.forces()
+        ctx.sync()
+        both = ctx.download(0).sorted_by_id()
+        two = ctx.download(1).sorted_by_id()
+    for f in ("rho", "nnb", "dvdt", "dudt", "dbdt", "divb", "dpsidt", "vsig"):
+        assert np.array_equal(getattr(alone, f), getattr(both, f)), f
+    compare_forces(two, oracle_stages(tcfg, tps, "mixed"), mhd=True)
+
+
 def test_hydro_run_batched_with_mhd_runs_equals_alone():
     """A hydro run in a context that carries MHD fields (another run is MHD) takes the MHD launch
     instantiation with its MHD terms off; its results equal the same run alone in a hydro context."""
