@@ -147,9 +147,9 @@ typedef struct sph_tuning {
                                FixedGrid policy, FrameDispatch.hpp:137-148) */
     int32_t max_lanes_per_target; /* cap on K, the lanes that share one target's candidates: 1..64 */
     int32_t pair_balance;   /* how the accepted pairs of a target are spread over its K lanes:
-                               0: each lane its own candidates; 1: compacted in LDS and shared
-                               round-robin; 2: whole-supercell neighbour lists (density_chunk 512,
-                               force_chunk 128) */
+                               0: each lane its own candidates; 1: the force pass compacts them in
+                               LDS and shares them round-robin (the density pass as 0); 2:
+                               whole-supercell neighbour lists (density_chunk 512, force_chunk 128) */
 } sph_tuning;
 
 typedef struct sph_ctx sph_ctx;

@@ -646,10 +646,10 @@ __device__ __forceinline__ void density_finish(const RunDev& R, int r, StateBuf&
 // 5% faster on the 2M-particle turbulence step. The other kernels keep the compiler's choice
 // (a cap spills them), as do the balanced pass and the 125-cell stencil with 256-candidate chunks,
 // whose LDS allows 6.
-template <int CH, bool MIXED, bool MHD, int KT, bool BAL, int RMAX>
+template <int CH, bool MIXED, bool MHD, int KT, int RMAX>
 __device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ runs, int nruns, int ncell_tot,
                                              int nblk_total, StateBuf& S, WorkBuf& W, int kmax, NbListR<RMAX>& L,
-                                             float4 (&sp)[CH], ListT<CH> (&lst)[BAL ? 64 * 32 : 1])
+                                             float4 (&sp)[CH])
 {
     constexpr int NS = RMAX == 1 ? 27 : 125;
     const int cell = xcd_cell(vb, nblk_total, ncell_tot);
@@ -730,27 +730,17 @@ __device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ 
                 if (!tgt) msk = 0;
                 if (keep) mw.put(msk, cbase + sb, lim);
                 cnt += (uint32_t)__popc(msk);
-                if (!BAL || K == 1) {
-                    const int kb = (int)sizeof(float4) * K;
-                    const int eb0 = (int)sizeof(float4) * (k + sb * K);  // byte offset of bit 0's slot
-                    while (msk) {
-                        const int ii = __builtin_ctz(msk);
-                        msk &= msk - 1;
-                        const float4 p = lds_at(sp, (int)__umul24((unsigned)ii, (unsigned)kb) + eb0);
-                        float dx, dy, dz;
-                        const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
-                        acc += p.w * kshape(kern, fast_sqrt(r2) * invh);
-                    }
-                } else {
-                    const int gtot = compact_group(msk, sb, lm, lst);
-                    __syncthreads();
-                    for (int q = k; q < gtot; q += K) {
-                        const float4 p = sp[lst[lm.gfirst * 32 + q]];
-                        float dx, dy, dz;
-                        const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
-                        acc += p.w * kshape(kern, fast_sqrt(r2) * invh);
-                    }
-                    __syncthreads();
+                // each lane sums its own accepted candidates (the density pairs are too cheap for
+                // sharing them among a target's lanes to pay for the compaction)
+                const int kb = (int)sizeof(float4) * K;
+                const int eb0 = (int)sizeof(float4) * (k + sb * K);  // byte offset of bit 0's slot
+                while (msk) {
+                    const int ii = __builtin_ctz(msk);
+                    msk &= msk - 1;
+                    const float4 p = lds_at(sp, (int)__umul24((unsigned)ii, (unsigned)kb) + eb0);
+                    float dx, dy, dz;
+                    const float r2 = geom_r2(own.x, own.y, own.z, p, dx, dy, dz);
+                    acc += p.w * kshape(kern, fast_sqrt(r2) * invh);
                 }
             }
             __syncthreads();
@@ -761,25 +751,23 @@ __device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ 
     }
 }
 
-template <int CH, bool MIXED, bool MHD, int KT, bool BAL, int RMAX = 1, bool GRID = false>
+template <int CH, bool MIXED, bool MHD, int KT, int RMAX = 1, bool GRID = false>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu((KT == 2 && !((BAL || RMAX == 2) && CH > 128)) ? 8 : 1)))  // KT 2: Wendland
+__attribute__((amdgpu_waves_per_eu((KT == 2 && !(RMAX == 2 && CH > 128)) ? 8 : 1)))  // KT 2: Wendland
 void k_density(const RunDev* __restrict__ runs, int nruns, int ncell_tot, int nblk_total, StateBuf S, WorkBuf W,
-               int kmax, int balance)
+               int kmax)
 {
     __shared__ NbListR<RMAX> L;
     __shared__ float4 sp[CH];
-    __shared__ ListT<CH> lst[BAL ? 64 * 32 : 1];  // compaction lists of the balanced pair loop
     if constexpr (GRID) {
         // a persistent grid: each wave walks the supercells at a stride of the grid (a multiple
         // of 8, so every cell a wave visits belongs to its own XCD's range)
         for (int vb = blockIdx.x; vb < nblk_total; vb += gridDim.x)
-            density_wave<CH, MIXED, MHD, KT, BAL, RMAX>(vb, runs, nruns, ncell_tot, nblk_total, S, W, kmax, L, sp,
-                                                        lst);
+            density_wave<CH, MIXED, MHD, KT, RMAX>(vb, runs, nruns, ncell_tot, nblk_total, S, W, kmax, L, sp);
     } else {
         // one supercell per wave
-        density_wave<CH, MIXED, MHD, KT, BAL, RMAX>((int)blockIdx.x, runs, nruns, ncell_tot, nblk_total, S, W, kmax,
-                                                    L, sp, lst);
+        density_wave<CH, MIXED, MHD, KT, RMAX>((int)blockIdx.x, runs, nruns, ncell_tot, nblk_total, S, W, kmax, L,
+                                               sp);
     }
 }
 
@@ -1036,7 +1024,7 @@ __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ r
                     }
                 }
                 if (!tgt) msk = 0;
-                if (BAL == 0 || K == 1) {
+                if (BAL == 0) {
                     const int kb = (int)sizeof(float4) * K;
                     const int eb0 = (int)sizeof(float4) * (k + sb * K);  // byte offset of bit 0's slot
                     while (msk) {
@@ -1056,12 +1044,16 @@ __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ r
                 } else {
                     const int gtot = compact_group(msk, sb, lm, lst);
                     __syncthreads();
+                    const ListT<CH>* mine = lst + lm.gfirst * 32;  // this target's accepted slots
                     for (int q = k; q < gtot; q += K) {
-                        const int e = lst[lm.gfirst * 32 + q];
+                        const int eb = (int)sizeof(float4) * (int)mine[q];
                         float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if constexpr (MHD) q4 = sp[4][e];
-                        force_pair<MHD>(o0, o1, o2, o3, o4, sp[0][e], sp[1][e], sp[2][e], sp[3][e], q4, kern, alpha,
-                                        beta, mhd, clean, A);
+                        if constexpr (MHD) q4 = lds_at(sp[4], eb);
+                        const float4 q0 = lds_at(sp[0], eb), q1 = lds_at(sp[1], eb), q2 = lds_at(sp[2], eb),
+                                     q3 = lds_at(sp[3], eb);
+                        __builtin_amdgcn_sched_barrier(0);
+                        force_pair<MHD, ONE_H>(o0, o1, o2, o3, o4, q0, q1, q2, q3, q4, kern, alpha, beta, mhd, clean,
+                                               A);
                     }
                     __syncthreads();
                 }
@@ -1073,8 +1065,11 @@ __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ r
 }
 
 template <int CH, bool MIXED, bool MHD, int KT, int BAL, int RMAX = 1, bool GRID = false, bool ONE_H = false>
-__global__ __launch_bounds__(64) void k_forces(const RunDev* __restrict__ runs, int nruns, int ncell_tot,
-                                               int nblk_total, StateBuf S, WorkBuf W, int kmax, int balance)
+__global__ __launch_bounds__(64)
+// the balanced loop fits 96 VGPRs without spilling (5 waves per SIMD; the compiler's own choice is 97)
+__attribute__((amdgpu_waves_per_eu(BAL == 1 && !GRID ? 5 : 1)))
+void k_forces(const RunDev* __restrict__ runs, int nruns, int ncell_tot, int nblk_total, StateBuf S, WorkBuf W,
+              int kmax, int balance)
 {
     constexpr int NP = MHD ? 5 : 4;
     __shared__ NbListR<RMAX> L;
@@ -1687,24 +1682,22 @@ hipError_t launch_scatter_gather(const LaunchCtx& c, const StateBuf& Snew)
         }                                                                                           \
     } while (0)
 
-template <int CH, bool BAL>
+template <int CH>
 static void density_ch(const LaunchCtx& c, int nblk)
 {
     const int grid = pair_grid(c, nblk);
     if (grid < nblk)
-        SPH_PAIR_DISPATCH(k_density, grid, CH, SPH_COMMA BAL SPH_COMMA 1 SPH_COMMA true, c.runs, c.nruns, c.ncell_tot,
-                          nblk, c.S, c.W, c.kmax, c.balance);
+        SPH_PAIR_DISPATCH(k_density, grid, CH, SPH_COMMA 1 SPH_COMMA true, c.runs, c.nruns, c.ncell_tot, nblk, c.S,
+                          c.W, c.kmax);
     else
-        SPH_PAIR_DISPATCH(k_density, nblk, CH, SPH_COMMA BAL, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax,
-                          c.balance);
+        SPH_PAIR_DISPATCH(k_density, nblk, CH, , c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
 }
 
 // supercells finer than H somewhere in the context: the chunked lane loop over the 125-cell stencil
 template <int CH>
 static void density_fine_ch(const LaunchCtx& c, int nblk)
 {
-    SPH_PAIR_DISPATCH(k_density, nblk, CH, SPH_COMMA false SPH_COMMA 2, c.runs, c.nruns, c.ncell_tot, nblk, c.S,
-                      c.W, c.kmax, 0);
+    SPH_PAIR_DISPATCH(k_density, nblk, CH, SPH_COMMA 2, c.runs, c.nruns, c.ncell_tot, nblk, c.S, c.W, c.kmax);
 }
 
 template <int CAPD>
@@ -1728,11 +1721,9 @@ hipError_t launch_density(const LaunchCtx& c)
     } else if (c.balance == 2) {
         density_cell_ch<512>(c, nblk);
     } else if (c.density_chunk >= 256) {
-        if (c.balance == 1) density_ch<256, true>(c, nblk);
-        else density_ch<256, false>(c, nblk);
+        density_ch<256>(c, nblk);
     } else {
-        if (c.balance == 1) density_ch<128, true>(c, nblk);
-        else density_ch<128, false>(c, nblk);
+        density_ch<128>(c, nblk);
     }
     return hipGetLastError();
 }
@@ -1744,7 +1735,7 @@ static void forces_ch(const LaunchCtx& c, int nblk)
     if (grid < nblk)
         SPH_PAIR_DISPATCH(k_forces, grid, CH, SPH_COMMA BAL SPH_COMMA 1 SPH_COMMA true, c.runs, c.nruns, c.ncell_tot,
                           nblk, c.S, c.W, c.kmax, c.balance);
-    else if (BAL == 0 && c.one_h)  // one smoothing length per run: the pair loop neither tests nor averages h
+    else if (c.one_h)  // one smoothing length per run: the pair loop neither tests nor averages h
         SPH_PAIR_DISPATCH(k_forces, nblk, CH, SPH_COMMA BAL SPH_COMMA 1 SPH_COMMA false SPH_COMMA true, c.runs, c.nruns,
                           c.ncell_tot, nblk, c.S, c.W, c.kmax, c.balance);
     else
