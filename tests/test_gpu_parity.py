@@ -285,8 +285,17 @@ def _big_cells_case():
     return cfg, ps
 
 
+def _two_h_case():
+    """MHD turbulence with every other particle at 0.9 h: pairs of unequal smoothing lengths take the
+    second kernel gradient and the mean h, in the general (not one-h) instantiation of each shape."""
+    cfg, ps = _turb(14, 19)
+    ps.h[1::2] = np.float32(0.9) * ps.h[1::2]
+    return cfg, ps
+
+
 CASES = {
     "turbulence": lambda: _turb(16, 17),
+    "two_h": _two_h_case,
     "sod_walls": lambda: ic.sod_tube(seed=5, nx_left=32, cross=4),
     "clump": _clump_case,
     "big_cells": _big_cells_case,
