@@ -889,17 +889,24 @@ __device__ __forceinline__ void force_write(const RunDev& R, WorkBuf& W, const F
     }
 }
 
-// K partial force sums of a target combined in lane order (deterministic), field by field; then
-// one lane per target writes its derivatives, signal speed and div B.
+// x from the lane D below in the same 16-lane row (DPP row_shr:D; lanes at the row's start read 0)
+template <int D>
+__device__ __forceinline__ float row_shr(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x110 + D, 0xf, 0xf, true));
+}
+
+// K partial force sums of a target combined (deterministic), field by field; then one lane per
+// target writes its derivatives, signal speed and div B. For K = 2, 4, 8 or 16 a target's lanes lie
+// in one 16-lane row, and log2 K DPP steps (each lane adds the partial of the lane 1, 2, 4, ... below
+// it) leave the target's total in its last lane; other K take the shuffles of each lane's partial
+// in lane order to every lane of the target.
 template <bool MHD>
 __device__ __forceinline__ void force_finish(const RunDev& R, WorkBuf& W, const LaneMap& lm, ForceAcc& A, bool tgt,
                                              int i, const float4& o1, const float4& o2, const float4& o3,
                                              const float4& o4, bool mhd, bool clean, float borve, float ch)
 {
     const int K = lm.K, k = lm.k;
-    // combine the K partial sums in lane order (deterministic). All fields of one lane step are
-    // fetched together, so a step's shuffles overlap instead of each waiting for the last (5% of
-    // the force pass at 2M particles against a loop per field; the additions are the same).
     constexpr int NF = MHD ? 17 : 5;
     float v[NF] = {A.ax, A.ay, A.az, A.DV, A.AVU};
     if constexpr (MHD) {
@@ -908,18 +915,42 @@ __device__ __forceinline__ void force_finish(const RunDev& R, WorkBuf& W, const 
         for (int f = 0; f < 12; ++f) v[5 + f] = m[f];
     }
     float s[NF];
+    float vm;
+    bool writer;
+    if (K == 1 || K == 2 || K == 4 || K == 8 || K == 16) {
+        auto step = [&](auto d) {
+            constexpr int D = decltype(d)::value;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) s[f] = 0.0f;
-    float vm = 0.0f;
-    for (int kk = 0; kk < K; ++kk) {
-        const int src = lm.gfirst + kk < 64 ? lm.gfirst + kk : 63;
-        float t[NF];
+            for (int f = 0; f < NF; ++f) v[f] += row_shr<D>(v[f]);
+            // signal speeds are >= +0: an integer max on their bits
+            A.vmax = __int_as_float(max(__float_as_int(A.vmax), __float_as_int(row_shr<D>(A.vmax))));
+        };
+        if (K >= 2) step(std::integral_constant<int, 1>{});
+        if (K >= 4) step(std::integral_constant<int, 2>{});
+        if (K >= 8) step(std::integral_constant<int, 4>{});
+        if (K >= 16) step(std::integral_constant<int, 8>{});
 #pragma unroll
-        for (int f = 0; f < NF; ++f) t[f] = __shfl(v[f], src, 64);
-        const float tm = __shfl(A.vmax, src, 64);
+        for (int f = 0; f < NF; ++f) s[f] = v[f];
+        vm = A.vmax;
+        writer = k == K - 1;
+    } else {
+        // each lane's partials shuffled to every lane of the target, added in lane order. All fields
+        // of one lane step are fetched together, so a step's shuffles overlap instead of each waiting
+        // for the last.
 #pragma unroll
-        for (int f = 0; f < NF; ++f) s[f] += t[f];
-        vm = fmaxf(vm, tm);
+        for (int f = 0; f < NF; ++f) s[f] = 0.0f;
+        vm = 0.0f;
+        for (int kk = 0; kk < K; ++kk) {
+            const int src = lm.gfirst + kk < 64 ? lm.gfirst + kk : 63;
+            float t[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) t[f] = __shfl(v[f], src, 64);
+            const float tm = __shfl(A.vmax, src, 64);
+#pragma unroll
+            for (int f = 0; f < NF; ++f) s[f] += t[f];
+            vm = fmaxf(vm, tm);
+        }
+        writer = k == 0;
     }
     ForceAcc& T = A;
     T.ax = s[0];
@@ -942,7 +973,7 @@ __device__ __forceinline__ void force_finish(const RunDev& R, WorkBuf& W, const 
         T.PSy = s[15];
         T.PSz = s[16];
     }
-    if (k == 0 && tgt) force_write<MHD>(R, W, T, i, o1, o2, o3, o4, mhd, clean, borve, ch);
+    if (writer && tgt) force_write<MHD>(R, W, T, i, o1, o2, o3, o4, mhd, clean, borve, ch);
 }
 
 template <int CH, bool MIXED, bool MHD, int KT, int BAL, int RMAX, bool ONE_H = false>
