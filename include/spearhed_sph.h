@@ -275,7 +275,9 @@ int sph_prime(sph_ctx* ctx);
 int sph_step(sph_ctx* ctx, int32_t n_steps);
 /* Capture two whole steps of every run into a hipGraph (a step swaps the double-buffered state,
  * so two return it to where capture began); sph_graph_launch runs n_steps steps by replaying it,
- * taking one step eagerly where the parity or an odd count needs it. Results equal sph_step's. */
+ * taking one step eagerly where the parity or an odd count needs it. Results equal sph_step's.
+ * The graph holds the force pass's variant for runs whose particles share one smoothing length; an
+ * upload that changes whether every run does so drops the graph (capture again). */
 int sph_graph_capture(sph_ctx* ctx);
 int sph_graph_launch(sph_ctx* ctx, int32_t n_steps);
 /* Per-particle sort results of the last resort (run-local, current order). */

@@ -311,6 +311,20 @@ void invalidate_graph(sph_ctx* c)
     c->graph_parity = -1;
 }
 
+// A run's smoothing-length flag after an upload or fill. The force pass's kernel variant follows
+// the flags of all runs; a captured graph holds the variant it was captured with, so it is dropped
+// when that changes (sph_graph_launch then reports that no graph is captured).
+void set_one_h(sph_ctx* c, int run, bool one)
+{
+    bool before = c->nruns > 0, after = c->nruns > 0;
+    for (int r = 0; r < c->nruns; ++r) {
+        before = before && c->one_h[r];
+        after = after && (r == run ? one : c->one_h[r] != 0);
+    }
+    c->one_h[run] = one ? 1 : 0;
+    if (before != after) invalidate_graph(c);
+}
+
 int do_bin(sph_ctx* c, double dt, int drift)
 {
     if (c->nruns == 0) return SPH_OK;  // an empty pool: nothing to launch
@@ -975,7 +989,7 @@ int sph_upload(sph_ctx* c, int32_t run, const sph_host_soa* s)
             return fail(c, SPH_E_INVALID, "upload: smoothing_length must be finite and > 0");
         one_h = one_h && s->smoothing_length[i] == s->smoothing_length[0];
     }
-    c->one_h[run] = one_h ? 1 : 0;
+    set_one_h(c, run, one_h);
     HIPTRY(c, hipStreamSynchronize(c->stream));
     const int64_t n = s->n, o = R.p0;
     StateBuf& S = c->S[c->cur];
@@ -1036,7 +1050,7 @@ int sph_init_run(sph_ctx* c, int32_t run, const sph_lattice_block* blocks, int32
     HIPTRY(c, launch_init_lattice(blocks, n_blocks, R.np, R.p0, seed, jitter, cfg.h0, cfg.gamma, c->mixed ? 1 : 0,
                                   c->has_mhd ? 1 : 0, c->S[c->cur], c->stream));
     // as an upload: time restarts at 0 and the next step primes the run (enqueued, nothing waits)
-    c->one_h[run] = 1;  // the fill gives every particle the run's h0
+    set_one_h(c, run, true);  // the fill gives every particle the run's h0
     HIPTRY(c, launch_restart_run(c->runs_d, c->W, run, c->stream));
     if (c->binned) HIPTRY(c, launch_fill_u32(c->W.count, 0u, c->ncell_tot, c->stream));
     c->binned = false;

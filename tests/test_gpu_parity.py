@@ -200,6 +200,36 @@ def test_graph_launch_of_zero_steps_changes_nothing():
         ctx.sync()
 
 
+def test_graph_dropped_when_the_force_variant_changes():
+    """A graph captured while every particle shares one smoothing length holds the one-h force
+    variant. Uploading particles of two smoothing lengths drops it (replay would be wrong); a graph
+    captured again replays like eager steps."""
+    cfg, ps = _turb(10, 25)
+    two = ps.copy()
+    two.h[1::2] = np.float32(0.9) * two.h[1::2]
+    outs = []
+    with Context([cfg], [ps.n], precision="mixed") as ctx:
+        ctx.upload(0, ps)
+        ctx.prime()
+        ctx.graph_capture()
+        ctx.upload(0, two)
+        with pytest.raises(capi.SphError, match="no captured graph"):
+            ctx.graph_launch(2)
+        ctx.prime()
+        ctx.graph_capture()
+        ctx.graph_launch(3)
+        ctx.sync()
+        outs.append(ctx.download(0).sorted_by_id())
+    with Context([cfg], [ps.n], precision="mixed") as ctx:
+        ctx.upload(0, two)
+        ctx.prime()
+        ctx.step(3)
+        ctx.sync()
+        outs.append(ctx.download(0).sorted_by_id())
+    for f in ("pos", "vel", "u", "rho", "B", "psi", "dvdt"):
+        assert np.array_equal(getattr(outs[0], f), getattr(outs[1], f)), f
+
+
 def test_batched_runs_equal_alone():
     """Each run of a ragged batch gives bit-identical results to the same run alone."""
     runs = [ic.kelvin_helmholtz(nt, 100 + r) for r, nt in enumerate([4200, 9000, 5100])]
