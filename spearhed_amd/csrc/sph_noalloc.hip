@@ -266,7 +266,7 @@ extern "C" int sph_density_noalloc(const sph_run_config* cfg, int32_t precision,
     W.perm = static_cast<int32_t*>(at(kReg.perm));
     // nbmask stays null: the density pass keeps no candidate masks (there is no force pass here)
 
-    LaunchCtx L;
+    LaunchCtx L{};
     L.runs = static_cast<const RunDev*>(at(kReg.run));
     L.nruns = 1;
     L.ncell_tot = R.ncell;
@@ -278,6 +278,8 @@ extern "C" int sph_density_noalloc(const sph_run_config* cfg, int32_t precision,
     L.kmax = 64;
     L.balance = 0;
     L.kern = cfg->kernel;
+    L.rmax = R.reach;  // 2 for supercells finer than H: the 125-supercell stencil
+    L.one_h = 0;
     L.persistent_waves = 0;
     L.S = A;
     L.W = W;

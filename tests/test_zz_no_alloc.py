@@ -118,6 +118,18 @@ def test_noalloc_two_calls_match_context(precision):
         assert nnb.min() > 20 and np.all(rho > 0)
 
 
+def test_noalloc_half_h_supercells_match_context():
+    """Supercells of edge H/2 (cell_factor 0.5): the no-alloc pass takes the 125-supercell stencil as
+    the context does, with the same bits."""
+    cfg, ps = _case("turbulence_small")
+    cfg.cell_factor = 0.5
+    print(f"noalloc half-H n={ps.n}", flush=True)
+    rho, nnb = _noalloc(cfg, ps, "mixed")
+    rho_c, nnb_c = _context(cfg, ps, "mixed")
+    assert np.array_equal(nnb, nnb_c) and np.array_equal(rho.view(np.uint32), rho_c.view(np.uint32))
+    assert nnb.min() > 20
+
+
 def test_noalloc_walls_report_targets_only():
     cfg, ps = _case("sod_walls")
     print(f"noalloc sod_walls n={ps.n}", flush=True)
