@@ -273,7 +273,7 @@ size_t stage_half_bytes(int cap_r, int cap_kb)
 
 LaunchCtx lctx(sph_ctx* c)
 {
-    LaunchCtx L;
+    LaunchCtx L{};
     L.runs = c->runs_d;
     L.nruns = c->nruns;
     L.ncell_tot = c->ncell_tot;

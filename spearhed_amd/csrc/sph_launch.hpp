@@ -17,7 +17,8 @@ struct LaunchCtx {
     int density_chunk;
     int force_chunk;
     int kmax;     // max lanes per target in the pair passes
-    int balance;  // share a target's accepted pairs among its K lanes
+    int balance;  // pair_balance: 1 = the force pass shares a target's accepted pairs among its K lanes,
+                  // 2 = whole-supercell neighbour lists
     int kern;     // smoothing kernel of every run (SPH_KERNEL_*), or -1 when runs differ
     int one_h;    // every run's particles share one smoothing length (the run's own)
     int rmax;     // largest stencil reach of the runs: 1 (supercells of edge >= H) or 2 (edge >= H/2; pair
