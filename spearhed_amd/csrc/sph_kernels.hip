@@ -252,8 +252,23 @@ __global__ __launch_bounds__(256) void k_gather(const RunDev* __restrict__ runs,
     if (local == 0) W.chbits[r] = 0u;  // cleaning-speed maximum of the coming density pass
     const uint32_t i = W.tmp[q];
     const uint32_t c = W.key[i];
-    const uint32_t cb = W.cs[c], ce = W.cs[c + 1];
     const uint64_t myid = W.tmpid[q];
+    // the particle's old record is read before the rank search, so its loads overlap the search's
+    const uint8_t live = Sold.mask[i], sp = Sold.species[i];
+    const float m = Sold.mass[i], hh = Sold.h[i], uu = Sold.u[i], rh = Sold.rho[i];
+    float vel[3], bf[3] = {0.f, 0.f, 0.f}, ps = 0.f;
+    double x64[3] = {0.0, 0.0, 0.0};
+    float x32[3] = {0.f, 0.f, 0.f};
+    for (int a = 0; a < 3; ++a) {
+        vel[a] = Sold.vel[a][i];
+        if (MIXED) x64[a] = Sold.pos64[a][i];
+        else x32[a] = Sold.pos32[a][i];
+    }
+    if (has_mhd) {
+        for (int a = 0; a < 3; ++a) bf[a] = Sold.B[a][i];
+        ps = Sold.psi[i];
+    }
+    const uint32_t cb = W.cs[c], ce = W.cs[c + 1];
     // rank within the supercell = number of smaller ids in it; four independent loads per round
     uint32_t rank = 0, mm = cb;
     for (; mm + 4 <= ce; mm += 4) {
@@ -266,33 +281,30 @@ __global__ __launch_bounds__(256) void k_gather(const RunDev* __restrict__ runs,
     if (W.perm) W.perm[f] = (int32_t)i;
 
     Snew.id[f] = myid;
-    const uint8_t live = Sold.mask[i], sp = Sold.species[i];
     Snew.mask[f] = live;
     Snew.species[f] = sp;
-    const float m = Sold.mass[i];
     Snew.mass[f] = m;
-    Snew.h[f] = Sold.h[i];
-    Snew.u[f] = Sold.u[i];
-    Snew.rho[f] = Sold.rho[i];
+    Snew.h[f] = hh;
+    Snew.u[f] = uu;
+    Snew.rho[f] = rh;
     float pc[3];
     const int lc = (int)c - R.c0;
     const int cxyz[3] = {lc % R.n[0], (lc / R.n[0]) % R.n[1], lc / (R.n[0] * R.n[1])};
     for (int a = 0; a < 3; ++a) {
 #pragma clang fp contract(off)
-        Snew.vel[a][f] = Sold.vel[a][i];
+        Snew.vel[a][f] = vel[a];
         if (MIXED) {
-            double x = Sold.pos64[a][i];
+            const double x = x64[a];
             Snew.pos64[a][f] = x;
             pc[a] = (float)((x - R.lo[a]) - (double)cxyz[a] * R.s[a]);
         } else {
-            float x = Sold.pos32[a][i];
-            Snew.pos32[a][f] = x;
-            pc[a] = x;
+            Snew.pos32[a][f] = x32[a];
+            pc[a] = x32[a];
         }
     }
     if (has_mhd) {
-        for (int a = 0; a < 3; ++a) Snew.B[a][f] = Sold.B[a][i];
-        Snew.psi[f] = Sold.psi[i];
+        for (int a = 0; a < 3; ++a) Snew.B[a][f] = bf[a];
+        Snew.psi[f] = ps;
     }
     float4 p0 = is_source(sp, live) ? make_float4(pc[0], pc[1], pc[2], m)
                                     : make_float4(kSentinel, kSentinel, kSentinel, 0.0f);
