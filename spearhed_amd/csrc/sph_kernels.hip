@@ -432,8 +432,10 @@ __device__ __forceinline__ int nb_segment(const NbListT<NS>& L, int v)
 // others in the same order, so the force pass replays the density pass's masks slot by slot as
 // before. With supercells of edge ~H about half of the 27-cell stencil goes (MHD turbulence at 2M
 // particles: 381 -> 194 candidates per supercell in a CPU replay), which halves the cull and the
-// staging and leaves fewer, fuller chunks for the pair loops (force pass 2.5% faster; the density
-// pass, bound by latency more than by its cull, is unchanged).
+// staging and leaves fewer, fuller chunks for the pair loops. The density pass filters; the force
+// pass reads the kept list back from the mask row (recomputing it only where it was not stored),
+// 18% faster than with no filter; the density pass, bound by latency more than by its cull, is 3%
+// slower.
 // ------------------------------------------------------------------------------------------------
 constexpr int kPreCap = 768;  // larger stencils, and supercells of more than 64 particles, stage whole
 
@@ -619,21 +621,38 @@ __device__ __forceinline__ LaneMap lane_map(int nt, int kmax)
 }
 
 // Whether a supercell-wave keeps its masks: one target group and every lane's candidates fit.
-__device__ __forceinline__ bool masks_fit(int nt, int total, const LaneMap& m)
+__device__ __forceinline__ bool masks_fit(int nt, int total, const LaneMap& m, int words = kMaskWords)
 {
-    return nt <= m.tpg && (total + m.K - 1) / m.K <= 32 * kMaskWords;
+    return nt <= m.tpg && (total + m.K - 1) / m.K <= 32 * words;
+}
+
+// The pre-filter's kept list travels from the density pass to the force pass in the top of the
+// supercell's mask row (kMaskWords x 64 words, seen as 16-bit entries): entry 1023 holds the kept
+// count (or kNoList), entries 1022, 1021, ... the kept list indices. With the filter on, masks use at
+// most kMaskWords - 1 words of each lane, so the count never shares a word with mask bits, and the
+// list is stored only where it stays above the words the masks use.
+constexpr uint16_t kNoList = 0xffffu;
+constexpr int kRow16 = kMaskWords * 64 * 2;  // 16-bit entries per mask row
+
+__device__ __forceinline__ bool list_fits(int count, int K)
+{
+    const int mask_words = ((count + K - 1) / K + 31) / 32;
+    return count <= kRow16 - 1 - 128 * mask_words;
 }
 
 // Mask words are indexed by lane candidate c (candidate v = c*K + k of sub-lane k), so the two
 // passes may stage with different chunk sizes. Bits for [c0, c0 + lim) may straddle two words.
+// Words at or past `wlim` (those of candidates past the lane's last one: the last chunk's padding,
+// all-zero bits) are not written, so the row above the lane's masks stays free.
 struct MaskWriter {
-    uint32_t* row;
-    uint32_t pend;
-    int pend_w;
+    uint32_t* row = nullptr;
+    uint32_t pend = 0;
+    int pend_w = -1;
+    int wlim = kMaskWords;  // <= kMaskWords
     __device__ __forceinline__ void put(uint32_t bits, int c0, int lim)
     {
         const int w = c0 >> 5, off = c0 & 31;
-        if (w >= kMaskWords) return;
+        if (w >= wlim) return;
         if (w != pend_w) {
             if (pend_w >= 0) row[pend_w * 64] = pend;
             pend = 0;
@@ -643,7 +662,7 @@ struct MaskWriter {
         if (off + lim > 32) {
             row[w * 64] = pend;
             pend = bits >> (32 - off);
-            pend_w = w + 1 < kMaskWords ? w + 1 : -1;
+            pend_w = w + 1 < wlim ? w + 1 : -1;
         }
     }
     __device__ __forceinline__ void flush()
@@ -780,11 +799,13 @@ __device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ 
     const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
     const float H2 = R.H2;
     const float gam = R.gamma;
-    const bool keep = masks_fit(nt, total, lm) && W.nbmask != nullptr;  // no mask table: density only
+    // no mask table: density only; with the filter on the top word of each lane carries the kept list
+    const bool keep = masks_fit(nt, total, lm, pre ? kMaskWords - 1 : kMaskWords) && W.nbmask != nullptr;
     MaskWriter mw;
     mw.row = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
     mw.pend = 0;
     mw.pend_w = -1;
+    mw.wlim = ((total + K - 1) / K + 31) / 32;  // the words the lane's candidates need
 
     for (int tg = 0; tg < nt; tg += lm.tpg) {
         const int t = tg + lm.tl;
@@ -858,6 +879,15 @@ __device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ 
         if (keep) mw.flush();
         density_finish<MHD>(R, r, S, W, lm, acc, cnt, valid, tgt, spc, live, i, own.w, h, invh, uu, vx, vy, vz,
                             rho_in, bx, by, bz, psi, gam, kern);
+    }
+    // the kept list for the force pass, stored last: stores count in the same wait counter as the
+    // loads, so stored earlier they would delay every later wait for the staging loads
+    if (pre && W.nbmask != nullptr) {
+        uint16_t* row16 = reinterpret_cast<uint16_t*>(W.nbmask + (size_t)cell * (kMaskWords * 64));
+        const bool store = keep && list_fits(total, K);
+        if (store)
+            for (int e = lane; e < total; e += 64) row16[kRow16 - 2 - e] = P.v[e];
+        if (lane == 0) row16[kRow16 - 1] = store ? (uint16_t)total : kNoList;
     }
 }
 
@@ -1103,12 +1133,25 @@ __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ r
     const RunDev& R = runs[r];
     build_nblist<MIXED, RMAX>(R, cell - R.c0, W.cs, L);
     const int self_v0 = L.off[self_segment<RMAX>(R)];
-    bool pre;
-    const int total = prefilter<NS>(L, S, W, t0, nt, L.off[NS], self_v0, R.H2, P, pre);  // the density pass's kept set
-    const StageOrder so(total);  // the density pass's staged order: its masks replay slot by slot
     const int lane = threadIdx.x;
     const LaneMap lm = lane_map(nt, kmax);
     const int K = lm.K, k = lm.k;
+    // the density pass's kept set: read back from the mask row where it stored it, else recomputed
+    bool pre = nt <= 64 && L.off[NS] <= kPreCap;
+    bool stored = false;
+    int total = L.off[NS];
+    if (pre && W.nbmask != nullptr) {
+        const uint16_t* row16 = reinterpret_cast<const uint16_t*>(W.nbmask + (size_t)cell * (kMaskWords * 64));
+        const int n = row16[kRow16 - 1];
+        if (n != kNoList) {
+            for (int e = lane; e < n; e += 64) P.v[e] = row16[kRow16 - 2 - e];
+            __syncthreads();
+            total = n;
+            stored = true;
+        }
+    }
+    if (!stored) total = prefilter<NS>(L, S, W, t0, nt, total, self_v0, R.H2, P, pre);
+    const StageOrder so(total);  // the density pass's staged order: its masks replay slot by slot
     const int per = CH / K;
     const int chk = per * K;
     const int kern = KT >= 0 ? KT : R.kernel;  // compile-time when every run uses one kernel
@@ -1118,7 +1161,7 @@ __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ r
     const bool clean = mhd && R.cleaning;
     const float borve = mhd ? R.borve : 0.0f;
     const float ch = __uint_as_float(W.chbits[r]);
-    const bool keep = masks_fit(nt, total, lm);
+    const bool keep = masks_fit(nt, total, lm, pre ? kMaskWords - 1 : kMaskWords);
     const int per_lane = (total + K - 1) / K;
     const uint32_t* mrow = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
 
@@ -1130,7 +1173,8 @@ __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ r
         const float4 o0 = W.P0[i], o1 = W.P1[i], o2 = W.P2[i], o3 = W.P3[i];
         float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f);
         if (MHD) o4 = W.P4[i];
-        const int self_u = tgt ? so.slot(pre ? (int)P.self_s[t] : self_v0 + t) : -1;
+        // the self slot is needed only where the pass culls (a stored list implies stored masks)
+        const int self_u = tgt && !keep ? so.slot(pre ? (int)P.self_s[t] : self_v0 + t) : -1;
         ForceAcc A = {};
 
         for (int base = 0; base < total; base += chk) {
@@ -1302,6 +1346,7 @@ __global__ __launch_bounds__(64) void k_density_cell(const RunDev* __restrict__ 
     mw.row = W.nbmask + (size_t)cell * (kMaskWords * 64) + lane;
     mw.pend = 0;
     mw.pend_w = -1;
+    mw.wlim = kMaskWords;
 
     for (int tg = 0; tg < nt; tg += lm.tpg) {
         const int t = tg + lm.tl;
