@@ -434,7 +434,7 @@ __device__ __forceinline__ int nb_segment(const NbListT<NS>& L, int v)
 // particles: 381 -> 194 candidates per supercell in a CPU replay), which halves the cull and the
 // staging and leaves fewer, fuller chunks for the pair loops. The density pass filters; the force
 // pass reads the kept list back from the mask row (recomputing it only where it was not stored),
-// 18% faster than with no filter; the density pass, bound by latency more than by its cull, is 3%
+// 11% faster than with no filter; the density pass, bound by latency more than by its cull, is 3%
 // slower.
 // ------------------------------------------------------------------------------------------------
 constexpr int kPreCap = 768;  // larger stencils, and supercells of more than 64 particles, stage whole
