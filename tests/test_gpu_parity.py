@@ -315,6 +315,34 @@ def _big_cells_case():
     return cfg, ps
 
 
+def _crowded_case():
+    """A periodic 20^3 lattice in 7^3 supercells (~23 particles each) with 12 and 6 extra particles
+    spread through two of them. Most supercells hand the force pass their kept candidate list in the
+    mask row; the one at ~29 particles (2 lanes per target) keeps too many candidates for the row, so
+    the force pass recomputes the list; the one at ~35 (1 lane per target) has more kept candidates
+    than its masks hold, so the force pass culls again with the recomputed list."""
+    n, h0 = 20, 0.065
+    dx = 1.0 / n
+    rng = np.random.default_rng(53)
+    lat = ic.lattice(n, n, n, (0, 0, 0), dx, rng)
+    e = 1.0 / 7
+    a = np.array([[2.0], [3.0], [4.0]]) * e
+    b = np.array([[5.0], [1.0], [2.0]]) * e
+    pos = np.concatenate([lat, a + rng.uniform(0.05 * e, 0.95 * e, size=(3, 12)),
+                          b + rng.uniform(0.05 * e, 0.95 * e, size=(3, 6))], axis=1)
+    ps = ParticleSet(pos.shape[1])
+    ps.id = np.arange(ps.n, dtype=np.uint64)
+    ps.pos = pos
+    ps.mass = np.full(ps.n, dx ** 3, dtype=np.float32)
+    ps.h = np.full(ps.n, h0, dtype=np.float32)
+    ps.u = np.full(ps.n, 1.5, dtype=np.float32)
+    ps.rho = np.ones(ps.n, dtype=np.float32)
+    ps.vel = rng.normal(0, 0.1, size=(3, ps.n)).astype(np.float32)
+    ps.B = np.stack([np.full(ps.n, 0.1), rng.normal(0, 0.02, ps.n), np.zeros(ps.n)]).astype(np.float32)
+    cfg = capi.make_config(periodic=(1, 1, 1), kernel=capi.KERNEL_WENDLAND, h0=h0, mhd=1, cleaning=1, borve=1)
+    return cfg, ps
+
+
 def _two_h_case():
     """MHD turbulence with every other particle at 0.9 h: pairs of unequal smoothing lengths take the
     second kernel gradient and the mean h, in the general (not one-h) instantiation of each shape."""
@@ -328,6 +356,7 @@ CASES = {
     "two_h": _two_h_case,
     "sod_walls": lambda: ic.sod_tube(seed=5, nx_left=32, cross=4),
     "clump": _clump_case,
+    "crowded": _crowded_case,
     "big_cells": _big_cells_case,
 }
 
