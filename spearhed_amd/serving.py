@@ -17,6 +17,7 @@ flight, so the host's admission work overlaps the device's step). p50 / p99 come
 """
 from __future__ import annotations
 
+import collections
 import ctypes as C
 import heapq
 import math
@@ -34,6 +35,11 @@ from .stepper import Pool
 
 
 RESPONSE_RINGS = 3  # pinned response rings, each sized for the pool's whole capacity
+# Batched steps kept in flight while the host retires and admits runs. A ring's copy-outs are
+# harvested once the step they follow completes, STEPS_IN_FLIGHT rounds after they were enqueued, and
+# a ring is refilled RESPONSE_RINGS retiring rounds after its last use: the depth must stay below
+# the ring count.
+STEPS_IN_FLIGHT = 2
 
 
 def pool_capacity(budget: int, max_runs: int, cells_per_particle: float, precision: str = "mixed",
@@ -251,7 +257,7 @@ class Server:
         if not finished and not add:
             return
         t_m = time.perf_counter()
-        new_idx = self.pool.set_runs(keep, [(q.cfg, q.n) for q in add])  # synchronises the stream
+        new_idx = self.pool.set_runs(keep, [(q.cfg, q.n) for q in add])  # enqueued behind the steps in flight
         self.stats.move_ms.append((time.perf_counter() - t_m) * 1e3)
         t_f = time.perf_counter()
         # an admitted run's first batched step primes it (density, forces, first dt; no motion),
@@ -301,10 +307,11 @@ class Server:
         """Run until every request has finished. realtime=False admits a request as soon as the
         queue reaches it (arrival times only order the queue).
 
-        One step is kept in flight: each round retires and admits runs (copy-outs, run-set change and
-        device fills enqueued behind the step in flight), enqueues the next batched step, and only
-        then waits for the previous one, so that the host's admission work overlaps the device's
-        step. step_ms is each step's device time (HIP events around it); step_wall_ms the host-side
+        STEPS_IN_FLIGHT steps are kept in flight: each round retires and admits runs (copy-outs,
+        run-set change and device fills enqueued behind the steps in flight), enqueues the next
+        batched step, and only then waits for the oldest step beyond that depth, so that the host's
+        admission work overlaps the device's steps even when a round takes longer than one step.
+        step_ms is each step's device time (HIP events around it); step_wall_ms the host-side
         interval between consecutive step completions."""
         for q in requests or []:
             self.submit(q)
@@ -315,32 +322,33 @@ class Server:
             self.pool.profile_begin(sum(q[2].steps + 1 for q in self.queue) + 8)
         self._t0 = time.perf_counter()
         n_steps = 0
-        inflight = None  # (mark of the step, copy-outs enqueued before it)
+        flight = collections.deque()  # (mark of a step, copy-outs enqueued before it), oldest first
         last_done = None
-        while (self.queue or self.resident or inflight) and n_steps < max_steps:
+        while (self.queue or self.resident or flight) and n_steps < max_steps:
             t_adm = time.perf_counter()
             self._retire_and_admit(self.now() if realtime else float("inf"))
             self.stats.admit_ms.append((time.perf_counter() - t_adm) * 1e3)
-            launched = None
             if self.resident:
                 self.pool.step(1)
-                launched = (self.pool.mark(), self._pending)
+                flight.append((self.pool.mark(), self._pending))
                 self._pending = []
                 self.stats.batch_runs.append(len(self.resident))
                 self.stats.batch_particles.append(sum(r.req.n for r in self.resident))
                 for r in self.resident:
                     r.done += 1
                 n_steps += 1
-            if inflight is not None:
-                self.pool.wait_mark(inflight[0])  # the previous step (and what preceded it) is done
+            # wait for the oldest steps beyond the depth (and what preceded them); with no run
+            # resident, for all of them
+            while flight and (len(flight) > STEPS_IN_FLIGHT or not self.resident):
+                mark, pending = flight.popleft()
+                self.pool.wait_mark(mark)
                 now = time.perf_counter()
                 if last_done is not None:
                     self.stats.step_wall_ms.append((now - last_done) * 1e3)
                 last_done = now
-                self._ready += inflight[1]
-                self._harvest()  # copied while the step just launched runs
-            inflight = launched
-            if inflight is None:
+                self._ready += pending
+                self._harvest()  # copied while the steps still in flight run
+            if not self.resident:
                 self.pool.sync()
                 self._synced()
                 self._harvest()
@@ -349,8 +357,8 @@ class Server:
                     t_idle = time.perf_counter()
                     time.sleep(max(0.0, min(self.queue[0][0] - self.now(), 0.01)))
                     self.stats.idle_ms.append((time.perf_counter() - t_idle) * 1e3)
-        if inflight is not None:  # stopped by max_steps with a step in flight
-            self._pending = inflight[1] + self._pending
+        while flight:  # stopped by max_steps with steps in flight: their copy-outs stay pending
+            self._pending = flight.pop()[1] + self._pending
         self._retire_and_admit(float("-inf"))  # retire the last runs, admit nothing
         self.pool.sync()
         self._synced()
