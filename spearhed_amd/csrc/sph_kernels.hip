@@ -478,8 +478,10 @@ __device__ __forceinline__ int prefilter(const NbListT<NS>& L, const StateBuf& S
     const float lx = wave_min(has ? o.x : INFINITY), hx = wave_max(has ? o.x : -INFINITY);
     const float ly = wave_min(has ? o.y : INFINITY), hy = wave_max(has ? o.y : -INFINITY);
     const float lz = wave_min(has ? o.z : INFINITY), hz = wave_max(has ? o.z : -INFINITY);
-    // a margin of 2^-13 of H^2: the rounding of this test and of the cull's r2 (a few units in the
-    // last place) stays far inside it, so no candidate the cull would accept is dropped
+    // The test cannot drop a candidate the cull accepts, at any coordinate magnitude: per axis the
+    // rounded distance to the box is at most the rounded distance to any target in it (rounding is
+    // monotone), and both sums of squares round in the same fma order, so the cull's r2 is never
+    // below this d2. The 2^-13 margin on H^2 is kept on top of that.
     const float H2m = H2 * (1.0f + 1.0f / 8192.0f);
     int count = 0;
     // four candidates per lane and round, all four records requested before any is tested
