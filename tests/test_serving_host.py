@@ -131,3 +131,42 @@ def test_poisson_arrivals_rate():
     t = poisson_arrivals(20000, 50.0, seed=1)
     assert (t[1:] >= t[:-1]).all()
     assert abs(len(t) / t[-1] - 50.0) < 2.0
+
+
+class DepthPool(RecorderPool):
+    """A recorder that completes marks in order and records how many steps were in flight (launched,
+    not yet waited for) each time a step is launched."""
+
+    def __init__(self, *a, **k):
+        super().__init__(*a, **k)
+        self.outstanding, self.depths = [], []
+
+    def step(self, n):
+        self.depths.append(len(self.outstanding))
+        super().step(n)
+
+    def mark(self):
+        t = super().mark()
+        self.outstanding.append(t)
+        return t
+
+    def wait_mark(self, ticket):
+        super().wait_mark(ticket)
+        self.outstanding = [t for t in self.outstanding if t > ticket]
+
+    def sync(self):
+        self.outstanding = []
+
+
+def test_steps_in_flight_stay_within_the_depth():
+    from spearhed_amd.serving import STEPS_IN_FLIGHT, RESPONSE_RINGS
+    assert STEPS_IN_FLIGHT < RESPONSE_RINGS  # a ring's copy-outs are harvested before it is refilled
+    reqs = _requests(6, [3, 5, 2])
+    srv = Server(1 << 34, pool_factory=DepthPool)
+    srv.serve(reqs, realtime=False)
+    d = srv.pool.depths
+    assert max(d) == STEPS_IN_FLIGHT  # the pipeline fills, and never beyond its depth
+    assert srv.pool.outstanding == []
+    assert sorted(srv.results) == [r.rid for r in reqs]
+    for r in reqs:
+        assert srv.results[r.rid].steps == r.steps  # after its priming step
