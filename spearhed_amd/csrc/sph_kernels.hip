@@ -850,8 +850,11 @@ __device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ 
             }
             __syncthreads();
             const int cbase = base / K;
-            for (int sb = 0; sb < per; sb += 32) {
-                const int lim = per - sb < 32 ? per - sb : 32;
+            // lane slots of this chunk that hold candidates (the most any lane has): the last chunk's
+            // padding is not culled
+            const int nval = ((total - base < chk ? total - base : chk) + K - 1) / K;
+            for (int sb = 0; sb < nval; sb += 32) {
+                const int lim = nval - sb < 32 ? nval - sb : 32;
                 uint32_t msk = cull32(sp, k + sb * K, K, lim, own.x, own.y, own.z, H2);
                 // the self pair (r2 = 0) passes the cull: drop it here; its density term is the
                 // self term m_i W(0, h_i) added below, as DensityInitSelf does
