@@ -440,9 +440,33 @@ __device__ __forceinline__ int nb_segment(const NbListT<NS>& L, int v)
 constexpr int kPreCap = 768;  // larger stencils, and supercells of more than 64 particles, stage whole
 
 struct PreList {
-    uint16_t v[kPreCap];  // list indices of the kept candidates, in list order
+    uint16_t v[kPreCap];  // the kept candidates in list order: list index, and for the 27-cell stencil
+                          // the segment in bits 10..14 (so staging needs no search)
     uint16_t self_s[64];  // kept index of each target of the supercell (a target is always kept)
 };
+
+// A kept-list entry: the list index (< kPreCap <= 1024) and, for the 27-cell stencil, its segment.
+template <int NS>
+__device__ __forceinline__ uint16_t pre_entry(int v, int sg)
+{
+    return (uint16_t)(NS == 27 ? v | (sg << 10) : v);
+}
+
+// List index and stencil segment of staged candidate su: from the kept list when the filter is on
+// (the segment packed in the entry for the 27-cell stencil, else searched), else su itself.
+template <int NS>
+__device__ __forceinline__ void staged_candidate(const NbListT<NS>& L, const PreList& P, bool pre, int su, int& v,
+                                                 int& sg)
+{
+    if (pre) {
+        const int e = P.v[su];
+        v = NS == 27 ? (e & 1023) : e;
+        sg = NS == 27 ? (e >> 10) : nb_segment(L, v);
+    } else {
+        v = su;
+        sg = nb_segment(L, v);
+    }
+}
 
 __device__ __forceinline__ float wave_min(float x)
 {
@@ -513,7 +537,7 @@ __device__ __forceinline__ int prefilter(const NbListT<NS>& L, const StateBuf& S
             if (keep) {
                 const int sidx = count + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
-                P.v[sidx] = (uint16_t)v;
+                P.v[sidx] = pre_entry<NS>(v, sg[r]);
                 const unsigned ts = (unsigned)(v - self_v0);
                 if (ts < (unsigned)nt) P.self_s[ts] = (uint16_t)sidx;
             }
@@ -838,9 +862,8 @@ __device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ 
                 const int u = base + e;
                 float4 p = make_float4(kSentinel, kSentinel, kSentinel, 0.0f);
                 if (u < total) {
-                    const int su = so.list_index(u);
-                    const int v = pre ? (int)P.v[su] : su;
-                    int sg = nb_segment(L, v);
+                    int v, sg;
+                    staged_candidate<NS>(L, P, pre, so.list_index(u), v, sg);
                     p = W.P0[L.start[sg] + (v - L.off[sg])];
                     p.x += L.sh[sg][0];
                     p.y += L.sh[sg][1];
@@ -1186,10 +1209,9 @@ __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ r
             for (int e = lane; e < chk; e += 64) {
                 const int u = base + e;
                 if (u < total) {
-                    const int su = so.list_index(u);
-                    const int v = pre ? (int)P.v[su] : su;
-                    int sg = nb_segment(L, v);
-                    int j = L.start[sg] + (v - L.off[sg]);
+                    int v, sg;
+                    staged_candidate<NS>(L, P, pre, so.list_index(u), v, sg);
+                    const int j = L.start[sg] + (v - L.off[sg]);
                     float4 p = W.P0[j];
                     p.x += L.sh[sg][0];
                     p.y += L.sh[sg][1];
