@@ -1159,24 +1159,35 @@ __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ r
     if (nt == 0) return;
     const int r = find_run_by_cell(runs, nruns, cell);
     const RunDev& R = runs[r];
+    const int lane = threadIdx.x;
+    // the density pass's kept set, stored in the mask row: its count and first 256 entries are
+    // requested before the neighbour list is built, so their loads overlap its cell-start loads
+    // (they are used only if this step's filter applies to the supercell, which the list decides)
+    const uint16_t* row16 =
+        W.nbmask != nullptr ? reinterpret_cast<const uint16_t*>(W.nbmask + (size_t)cell * (kMaskWords * 64)) : nullptr;
+    int n_row = kNoList;
+    uint16_t head[4] = {0, 0, 0, 0};
+    if (row16 != nullptr && nt <= 64) {
+        n_row = row16[kRow16 - 1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) head[q] = row16[kRow16 - 2 - (lane + 64 * q)];
+    }
     build_nblist<MIXED, RMAX>(R, cell - R.c0, W.cs, L);
     const int self_v0 = L.off[self_segment<RMAX>(R)];
-    const int lane = threadIdx.x;
     const LaneMap lm = lane_map(nt, kmax);
     const int K = lm.K, k = lm.k;
-    // the density pass's kept set: read back from the mask row where it stored it, else recomputed
+    // read back where the density pass stored it, else recomputed
     bool pre = nt <= 64 && L.off[NS] <= kPreCap;
     bool stored = false;
     int total = L.off[NS];
-    if (pre && W.nbmask != nullptr) {
-        const uint16_t* row16 = reinterpret_cast<const uint16_t*>(W.nbmask + (size_t)cell * (kMaskWords * 64));
-        const int n = row16[kRow16 - 1];
-        if (n != kNoList) {
-            for (int e = lane; e < n; e += 64) P.v[e] = row16[kRow16 - 2 - e];
-            __syncthreads();
-            total = n;
-            stored = true;
-        }
+    if (pre && row16 != nullptr && n_row != kNoList) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (lane + 64 * q < n_row) P.v[lane + 64 * q] = head[q];
+        for (int e = lane + 256; e < n_row; e += 64) P.v[e] = row16[kRow16 - 2 - e];
+        __syncthreads();
+        total = n_row;
+        stored = true;
     }
     if (!stored) total = prefilter<NS>(L, S, W, t0, nt, total, self_v0, R.H2, P, pre);
     const StageOrder so(total);  // the density pass's staged order: its masks replay slot by slot
