@@ -482,26 +482,55 @@ __device__ __forceinline__ float wave_max(float x)
     return x;
 }
 
+// Bounding box of the targets among a supercell's nt <= 64 particles (from their position records;
+// empty, lo > hi, without targets). Its inputs need no neighbour list: a pass requests them first
+// (target_box_load) and reduces them once the list is built (target_box), so the loads overlap.
+struct TargetBox {
+    float lx, ly, lz, hx, hy, hz;
+};
+
+struct TargetBoxInput {
+    uint8_t species = 0, mask = 0;
+    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+};
+
+__device__ __forceinline__ TargetBoxInput target_box_load(const StateBuf& S, const WorkBuf& W, int t0, int nt)
+{
+    const int lane = threadIdx.x;
+    TargetBoxInput in;
+    if (lane < nt) {
+        const int i = t0 + lane;
+        in.species = S.species[i];
+        in.mask = S.mask[i];
+        in.o = W.P0[i];  // a target is a source: its record holds its position
+    }
+    return in;
+}
+
+__device__ __forceinline__ TargetBox target_box(const TargetBoxInput& in, int nt)
+{
+    const bool has = (int)threadIdx.x < nt && is_target(in.species, in.mask);
+    TargetBox b;
+    b.lx = wave_min(has ? in.o.x : INFINITY);
+    b.hx = wave_max(has ? in.o.x : -INFINITY);
+    b.ly = wave_min(has ? in.o.y : INFINITY);
+    b.hy = wave_max(has ? in.o.y : -INFINITY);
+    b.lz = wave_min(has ? in.o.z : INFINITY);
+    b.hz = wave_max(has ? in.o.z : -INFINITY);
+    return b;
+}
+
 // Keeps in P the candidates of the supercell's list (of `total`) that lie within the support radius
-// of its targets' bounding box, and returns how many there are; with the filter off (`on` false)
+// of its targets' bounding box B, and returns how many there are; with the filter off (`on` false)
 // returns `total` and leaves P unused.
 template <int NS>
-__device__ __forceinline__ int prefilter(const NbListT<NS>& L, const StateBuf& S, const WorkBuf& W, int t0, int nt,
+__device__ __forceinline__ int prefilter(const NbListT<NS>& L, const WorkBuf& W, const TargetBox& B, int nt,
                                          int total, int self_v0, float H2, PreList& P, bool& on)
 {
     on = nt <= 64 && total <= kPreCap;
     if (!on) return total;
     const int lane = threadIdx.x;
-    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
-    bool has = false;
-    if (lane < nt) {
-        const int i = t0 + lane;
-        has = is_target(S.species[i], S.mask[i]);
-        if (has) o = W.P0[i];  // a target is a source: its record holds its position
-    }
-    const float lx = wave_min(has ? o.x : INFINITY), hx = wave_max(has ? o.x : -INFINITY);
-    const float ly = wave_min(has ? o.y : INFINITY), hy = wave_max(has ? o.y : -INFINITY);
-    const float lz = wave_min(has ? o.z : INFINITY), hz = wave_max(has ? o.z : -INFINITY);
+    const float lx = B.lx, hx = B.hx, ly = B.ly, hy = B.hy, lz = B.lz, hz = B.hz;
     // The test cannot drop a candidate the cull accepts, at any coordinate magnitude: per axis the
     // rounded distance to the box is at most the rounded distance to any target in it (rounding is
     // monotone), and both sums of squares round in the same fma order, so the cull's r2 is never
@@ -812,10 +841,13 @@ __device__ __forceinline__ void density_wave(int vb, const RunDev* __restrict__ 
     if (nt == 0) return;
     const int r = find_run_by_cell(runs, nruns, cell);
     const RunDev& R = runs[r];
+    // the targets' box inputs are requested before the neighbour list, so their loads overlap the list's
+    const TargetBoxInput box_in = nt <= 64 ? target_box_load(S, W, t0, nt) : TargetBoxInput{};
     build_nblist<MIXED, RMAX>(R, cell - R.c0, W.cs, L);
     const int self_v0 = L.off[self_segment<RMAX>(R)];
     bool pre;
-    const int total = prefilter<NS>(L, S, W, t0, nt, L.off[NS], self_v0, R.H2, P, pre);  // staged candidates
+    const int total = prefilter<NS>(L, W, nt <= 64 ? target_box(box_in, nt) : TargetBox{}, nt, L.off[NS], self_v0,
+                                    R.H2, P, pre);  // staged candidates
     const StageOrder so(total);
     const int lane = threadIdx.x;
     const LaneMap lm = lane_map(nt, kmax);
@@ -1189,7 +1221,9 @@ __device__ __forceinline__ void forces_wave(int vb, const RunDev* __restrict__ r
         total = n_row;
         stored = true;
     }
-    if (!stored) total = prefilter<NS>(L, S, W, t0, nt, total, self_v0, R.H2, P, pre);
+    if (!stored)
+        total = prefilter<NS>(L, W, pre ? target_box(target_box_load(S, W, t0, nt), nt) : TargetBox{}, nt, total,
+                              self_v0, R.H2, P, pre);
     const StageOrder so(total);  // the density pass's staged order: its masks replay slot by slot
     const int per = CH / K;
     const int chk = per * K;
