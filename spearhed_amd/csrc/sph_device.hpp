@@ -107,10 +107,10 @@ __device__ __forceinline__ float kshape(int kern, float q)
 {
 #pragma clang fp contract(off)
     if (kern == 2) {  // Wendland C2: (1 - q/2)^4 (1 + 2q)
-        const float t = fmaf(-0.5f, q, 1.0f);
+        // 1 - q/2 clamped at 0: exactly 0 for q >= 2 (the support test), the same bits below it
+        const float t = fmaxf(fmaf(-0.5f, q, 1.0f), 0.0f);
         const float t2 = t * t;
-        const float w = (t2 * t2) * fmaf(2.0f, q, 1.0f);
-        return q < 2.0f ? w : 0.0f;
+        return (t2 * t2) * fmaf(2.0f, q, 1.0f);
     }
     if (kern == 0) {  // cubic M4: 1 - 3/2 q^2 + 3/4 q^3 inside, (2 - q)^3 / 4 outside
         const float inner = fmaf(q * q, fmaf(0.75f, q, -1.5f), 1.0f);
